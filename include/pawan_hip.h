@@ -1,0 +1,200 @@
+/*
+ * pawan_hip.h -- C ABI of the MI355X (gfx950) vortex-particle interaction and
+ * RK4 integration path.
+ *
+ * The entry points replace the compute behind pawan's C++ classes
+ * (namespace pawan):
+ *   __interaction  src/interaction/interaction.h:21-235
+ *   __parallel     src/interaction/parallel.h:20-145
+ *   __integration  src/integration/integration.h:20-58
+ *   __rk4          src/integration/rk4.h:14-41
+ * A C++ subclass of __interaction / __rk4 (see the adapter header
+ * pawan_gpu.hpp) forwards its virtuals here; nothing else in the driver
+ * changes.
+ *
+ * Conventions
+ *   - Plain pointers and sizes; no GSL or torch types.  Particle arrays are
+ *     the reference's storage: positions/vorticities N x 3 row-major doubles
+ *     (gsl_matrix data with tda = 3), radius/volume N doubles.
+ *   - State and rate vectors use the reference layout: for each wake in the
+ *     order added, [positions 3n | vorticities 3n] (wake.cpp:87-120).
+ *   - Every function returns PW_OK (0) or a PW_E* code; pw_last_error()
+ *     gives the message.  No C++ exception crosses the ABI.
+ *   - Host arrays are borrowed for the duration of the call only.
+ *   - A context owns one HIP stream and all of its device memory; use it
+ *     from one host thread at a time.  Distinct contexts are independent.
+ *   - Results are deterministic: each particle's sums are taken in an order
+ *     that depends only on its own simulation, never on other work sharing
+ *     the GPU.
+ */
+#ifndef PAWAN_HIP_H_
+#define PAWAN_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define PW_OK 0
+#define PW_EINVAL 1     /* bad argument or call order                        */
+#define PW_ENOMEM 2     /* device or host allocation failed / over budget    */
+#define PW_EHIP 3       /* HIP runtime error                                 */
+#define PW_ENONFINITE 4 /* a rate or state became inf/NaN (coincident pairs) */
+#define PW_ENOTFOUND 5  /* unknown job id                                    */
+
+/* smoothing kernel: interaction_utils.h:13-24 selects one at compile time */
+#define PW_KERNEL_HIGHORDER 0 /* highorder.h:20-60 (reference default)  */
+#define PW_KERNEL_GAUSSIAN 1  /* gaussian.h:20-58                      */
+
+/* arithmetic of the pair evaluation */
+#define PW_FP64 0  /* binary64 throughout (the reference method)              */
+#define PW_FP32C 1 /* binary32 pair terms, compensated per-target sums,
+                      binary64 cross-tile reduction, state and diagnostics    */
+
+typedef struct pw_config {
+    int device;       /* HIP device ordinal                                      */
+    int kernel;       /* PW_KERNEL_*                                             */
+    int precision;    /* PW_FP64 / PW_FP32C                                      */
+    double nu;        /* kinematic viscosity (reference: 2.5e-3, interaction.cpp:13) */
+    double uinf[3];   /* uniform freestream added to every velocity (0 = none)     */
+    int block;        /* targets per workgroup 64/128/256; 0 = tuned            */
+    int chunk;        /* sources per work item (multiple of block); 0 = tuned   */
+} pw_config;
+
+typedef struct pw_ctx pw_ctx;
+
+/* library identification: "pawan_hip <version> gfx950" */
+const char *pw_version(void);
+/* fill cfg with defaults: device 0, HIGHORDER, FP64, nu 2.5e-3, no freestream */
+int pw_config_default(pw_config *cfg);
+
+/* __interaction() (interaction.cpp:11-18) */
+int pw_create(pw_ctx **out, const pw_config *cfg);
+/* ~__interaction() (interaction.cpp:29-33) */
+int pw_destroy(pw_ctx *ctx);
+/* message for the last failing call on ctx (or on ctx == NULL: last global) */
+const char *pw_last_error(const pw_ctx *ctx);
+
+/* __interaction::_nu (interaction.h:24); the reference has no setter */
+int pw_set_nu(pw_ctx *ctx, double nu);
+
+/* __interaction::addWake (interaction.cpp:35-39).  Copies the particles of
+ * one wake (__wake::_position/_vorticity/_radius/_volume, wake.h:35-40) to
+ * the device.  Wakes interact with themselves and with each other, as
+ * interact() does (interaction.cpp:95-108). */
+int pw_add_wake(pw_ctx *ctx, size_t n, const double *pos, const double *vort,
+                const double *radius, const double *volume);
+/* __system::_size: length of the state vector, 6 * total particles */
+size_t pw_size(const pw_ctx *ctx);
+size_t pw_num_wakes(const pw_ctx *ctx);
+
+/* __interaction::setStates / getStates (interaction.cpp:204-229) */
+int pw_set_state(pw_ctx *ctx, const double *state);
+int pw_get_state(pw_ctx *ctx, double *state);
+/* __interaction::solve -> interact() (interaction.cpp:41-43,95-108):
+ * velocity and vorticity rate of every particle, kept on the device */
+int pw_solve(pw_ctx *ctx);
+/* __interaction::getRates (interaction.cpp:213-220): [vel 3n | dvort 3n] per wake */
+int pw_get_rates(pw_ctx *ctx, double *rates);
+/* one wake's __wake::_position/_vorticity and _velocity/_retvorcity (N x 3) */
+int pw_get_wake(pw_ctx *ctx, size_t wake, double *pos, double *vort, double *vel,
+                double *dvort);
+
+/* __rk4::step (rk4.cpp:12-77) repeated nsteps times on device-resident state */
+int pw_rk4_steps(pw_ctx *ctx, double dt, size_t nsteps);
+/* __integration::step (integration.cpp:37-45), forward Euler */
+int pw_euler_steps(pw_ctx *ctx, double dt, size_t nsteps);
+
+/* calculateTotalVorticity / calculateLinearImpulse (interaction.cpp:240-259)
+ * and calculateKineticEnergy (interaction.cpp:351-383) summed over wakes
+ * and wake pairs; any pointer may be NULL to skip that quantity */
+int pw_diagnostics(pw_ctx *ctx, double omega[3], double impulse[3], double *ke);
+
+/* device time of the last pw_solve / pw_rk4_steps / pw_euler_steps call,
+ * measured with HIP events on the context's stream: total ms and the summed
+ * ms of the pair-interaction kernel alone, plus the pairs it evaluated */
+int pw_last_timing(const pw_ctx *ctx, double *total_ms, double *pair_ms, double *pairs);
+
+/* ------------------------------------------------------------------ batch
+ * Many independent simulations stepped together: one persistent launch over
+ * a table of (job, target tile, source chunk) work items per RHS, and the RK4
+ * stage combinations fused with the per-target reduction.  The step is
+ * captured once as a HIP graph and replayed. */
+typedef struct pw_batch pw_batch;
+
+typedef struct pw_job_spec {
+    size_t n;             /* particles                                     */
+    const double *pos;    /* N x 3                                         */
+    const double *vort;   /* N x 3                                         */
+    const double *radius; /* N                                             */
+    const double *volume; /* N                                             */
+    double nu;            /* viscosity                                     */
+    double dt;            /* RK4 time step                                 */
+    double uinf[3];       /* freestream                                    */
+    int precision;        /* PW_FP64 / PW_FP32C                            */
+} pw_job_spec;
+
+int pw_batch_create(pw_batch **out, int device, int kernel, int block);
+int pw_batch_destroy(pw_batch *b);
+const char *pw_batch_last_error(const pw_batch *b);
+/* returns the job id in *id */
+int pw_batch_add(pw_batch *b, const pw_job_spec *spec, int64_t *id);
+int pw_batch_remove(pw_batch *b, int64_t id);
+/* RK4-advance every job by nsteps (use_graph: replay a captured step) */
+int pw_batch_step(pw_batch *b, size_t nsteps, int use_graph);
+/* evaluate rates only, for all jobs */
+int pw_batch_solve(pw_batch *b);
+/* one job's particle positions/vorticities and last rates (N x 3 each; NULL skips) */
+int pw_batch_get(pw_batch *b, int64_t id, double *pos, double *vort, double *vel, double *dvort);
+/* per-job status: PW_OK or PW_ENONFINITE after the last step */
+int pw_batch_status(pw_batch *b, int64_t id);
+int pw_batch_last_timing(const pw_batch *b, double *total_ms, double *pair_ms, double *pairs);
+/* source chunk length used for a job with n particles (a function of n only) */
+size_t pw_chunk_for(size_t n, int block);
+
+/* ---------------------------------------------------------------- serving
+ * A continuous-batching request loop: jobs are queued with an arrival time,
+ * admitted while their HBM footprint fits the budget, stepped together with
+ * every other resident job, rebatched every step as jobs join and finish, and
+ * retired with their final state kept until fetched. */
+typedef struct pw_server pw_server;
+
+typedef struct pw_server_stats {
+    int64_t submitted, admitted, finished, failed, cancelled, queued, running;
+    int64_t steps;             /* batched steps executed                      */
+    double pairs;              /* ordered pair-interactions evaluated         */
+    double busy_ms;            /* device time of the batched steps            */
+    double step_p50_ms, step_p99_ms; /* batched step latency                  */
+    double job_p50_ms, job_p99_ms;   /* arrival -> final state ready          */
+    size_t hbm_in_use, hbm_budget;   /* bytes                                 */
+} pw_server_stats;
+
+int pw_server_create(pw_server **out, int device, int kernel, size_t hbm_budget_bytes);
+int pw_server_destroy(pw_server *s);
+const char *pw_server_last_error(const pw_server *s);
+/* the server's clock: milliseconds since pw_server_create (steady clock) */
+double pw_server_now_ms(const pw_server *s);
+/* queue a job that runs nsteps RK4 steps; arrival_ms on the server clock
+ * (negative: now).  Job latency is measured from arrival to retirement. */
+int pw_server_submit(pw_server *s, const pw_job_spec *spec, size_t nsteps, double arrival_ms,
+                     int64_t *id);
+int pw_server_cancel(pw_server *s, int64_t id);
+/* one pass of the request loop: admit queued jobs that fit the budget, run
+ * one batched RK4 step over every resident job, retire finished and failed
+ * jobs.  *running receives the number of resident jobs after the pass. */
+int pw_server_step(pw_server *s, int64_t *running);
+/* 0 queued, 1 running, 2 done, 3 failed (non-finite), 4 cancelled, -1 unknown */
+int pw_server_job_state(pw_server *s, int64_t id, int64_t *steps_done);
+/* final positions/vorticities of a finished job (N x 3 each); frees it */
+int pw_server_fetch(pw_server *s, int64_t id, double *pos, double *vort);
+int pw_server_get_stats(pw_server *s, pw_server_stats *st);
+/* device bytes one job of n particles occupies while resident */
+size_t pw_job_footprint(size_t n, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

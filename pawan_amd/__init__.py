@@ -1,0 +1,14 @@
+"""pawan_amd: the vortex-particle interaction and RK4 path of pawan
+(punsingh/pawan) on one AMD MI355X (gfx950), behind pawan's interaction /
+integration API.
+
+The compute lives in ``libpawan_hip.so`` (hand-written HIP kernels and a C
+ABI, include/pawan_hip.h); this package is its host-side mirror of the
+reference's classes plus the seeded workload generators.
+"""
+from . import generators  # noqa: F401
+from ._lib import FP32C, FP64, KERNEL_GAUSSIAN, KERNEL_HIGHORDER, PawanError  # noqa: F401
+from .interaction import RK4, Batch, Integration, Interaction, Server, Wake  # noqa: F401
+
+__all__ = ["Interaction", "Integration", "RK4", "Batch", "Server", "Wake", "generators",
+           "FP64", "FP32C", "KERNEL_HIGHORDER", "KERNEL_GAUSSIAN", "PawanError"]

@@ -1,0 +1,609 @@
+// vpm_capi.hip -- the extern "C" surface of include/pawan_hip.h over the engine.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pawan_hip.h"
+#include "vpm_engine.hpp"
+
+using pw::Engine;
+using pw::Error;
+using pw::Job;
+using pw::JobParams;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(std::string *err, F &&f) {
+    try {
+        f();
+        return PW_OK;
+    } catch (const Error &e) {
+        if (err) *err = e.what();
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (err) *err = "host allocation failed";
+        g_last_error = "host allocation failed";
+        return PW_ENOMEM;
+    } catch (const std::exception &e) {
+        if (err) *err = e.what();
+        g_last_error = e.what();
+        return PW_EINVAL;
+    }
+}
+
+int default_block(size_t n) {
+    if (n >= 32768) return 256;
+    if (n >= 8192) return 128;
+    return 64;
+}
+
+bool finite_params(double nu, double dt, const double *u) {
+    return std::isfinite(nu) && std::isfinite(dt) && std::isfinite(u[0]) && std::isfinite(u[1]) &&
+           std::isfinite(u[2]);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- context
+struct pw_ctx {
+    pw_config cfg;
+    std::string err;
+    std::unique_ptr<Engine> eng;
+    Job *job = nullptr;
+    bool dirty = false;
+    // wakes as added (host copies; refreshed from the device before a rebuild)
+    std::vector<size_t> wake_n;
+    std::vector<int64_t> woff{0};
+    std::vector<double> pos, vort, rad, vol;
+    double dt = 0.0;
+
+    size_t total() const { return (size_t)woff.back(); }
+
+    void rebuild() {
+        if (total() == 0) throw Error(PW_EINVAL, "no particles: add a wake first");
+        int tb = cfg.block ? cfg.block : default_block(total());
+        if (!eng || eng->block() != tb) {
+            job = nullptr;
+            eng.reset();
+            eng.reset(new Engine(cfg.device, cfg.kernel, tb));
+        }
+        if (job) {
+            eng->remove_job(job->id);
+            job = nullptr;
+        }
+        JobParams par;
+        par.nu = cfg.nu;
+        par.dt = dt;
+        std::memcpy(par.uinf, cfg.uinf, sizeof par.uinf);
+        par.precision = cfg.precision;
+        job = eng->add_job(1, total(), pos.data(), vort.data(), rad.data(), vol.data(), par,
+                           (size_t)std::max(0, cfg.chunk));
+        dirty = false;
+    }
+    void ready() {
+        if (dirty || !job) rebuild();
+    }
+    void set_dt(double d) {
+        if (d != dt) {
+            dt = d;
+            JobParams par = job->par;
+            par.dt = d;
+            eng->set_params(job, par);
+        }
+    }
+    int flag_status() { return eng->take_flag(job) ? PW_ENONFINITE : PW_OK; }
+};
+
+extern "C" {
+
+const char *pw_version(void) { return "pawan_hip 1.0 gfx950"; }
+
+int pw_config_default(pw_config *cfg) {
+    if (!cfg) return PW_EINVAL;
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->device = 0;
+    cfg->kernel = PW_KERNEL_HIGHORDER;
+    cfg->precision = PW_FP64;
+    cfg->nu = 2.5e-3;
+    return PW_OK;
+}
+
+int pw_create(pw_ctx **out, const pw_config *cfg) {
+    if (!out) return PW_EINVAL;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        pw_config c;
+        if (cfg)
+            c = *cfg;
+        else
+            pw_config_default(&c);
+        if (c.kernel != PW_KERNEL_HIGHORDER && c.kernel != PW_KERNEL_GAUSSIAN)
+            throw Error(PW_EINVAL, "unknown kernel");
+        if (c.precision != PW_FP64 && c.precision != PW_FP32C)
+            throw Error(PW_EINVAL, "unknown precision");
+        if (c.block != 0 && c.block != 64 && c.block != 128 && c.block != 256)
+            throw Error(PW_EINVAL, "block must be 0, 64, 128 or 256");
+        if (c.chunk < 0 || (c.chunk && c.block && c.chunk % c.block))
+            throw Error(PW_EINVAL, "chunk must be a non-negative multiple of block");
+        if (!finite_params(c.nu, 0.0, c.uinf)) throw Error(PW_EINVAL, "non-finite parameter");
+        int ndev = 0;
+        pw::hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+        if (c.device < 0 || c.device >= ndev) throw Error(PW_EINVAL, "no such HIP device");
+        pw_ctx *x = new pw_ctx();
+        x->cfg = c;
+        *out = x;
+    });
+}
+
+int pw_destroy(pw_ctx *ctx) {
+    if (!ctx) return PW_EINVAL;
+    int rc = guarded(nullptr, [&] { ctx->eng.reset(); });
+    delete ctx;
+    return rc;
+}
+
+const char *pw_last_error(const pw_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : g_last_error.c_str();
+}
+
+int pw_set_nu(pw_ctx *ctx, double nu) {
+    if (!ctx) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        if (!std::isfinite(nu)) throw Error(PW_EINVAL, "non-finite viscosity");
+        ctx->cfg.nu = nu;
+        if (ctx->job) {
+            JobParams par = ctx->job->par;
+            par.nu = nu;
+            ctx->eng->set_params(ctx->job, par);
+        }
+    });
+}
+
+int pw_add_wake(pw_ctx *ctx, size_t n, const double *pos, const double *vort,
+                const double *radius, const double *volume) {
+    if (!ctx) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        if (n == 0 || !pos || !vort || !radius || !volume)
+            throw Error(PW_EINVAL, "empty wake or null array");
+        if (ctx->job) {  // keep the device state of the wakes already present
+            ctx->eng->download_aos(ctx->job, 0, ctx->pos.data());
+            ctx->eng->download_aos(ctx->job, 1, ctx->vort.data());
+        }
+        ctx->pos.insert(ctx->pos.end(), pos, pos + 3 * n);
+        ctx->vort.insert(ctx->vort.end(), vort, vort + 3 * n);
+        ctx->rad.insert(ctx->rad.end(), radius, radius + n);
+        ctx->vol.insert(ctx->vol.end(), volume, volume + n);
+        ctx->wake_n.push_back(n);
+        ctx->woff.push_back(ctx->woff.back() + (int64_t)n);
+        ctx->dirty = true;
+    });
+}
+
+size_t pw_size(const pw_ctx *ctx) { return ctx ? 6 * ctx->total() : 0; }
+size_t pw_num_wakes(const pw_ctx *ctx) { return ctx ? ctx->wake_n.size() : 0; }
+
+int pw_set_state(pw_ctx *ctx, const double *state) {
+    if (!ctx || !state) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        ctx->ready();
+        ctx->eng->upload_state_layout(ctx->job, state, ctx->woff);
+    });
+}
+
+int pw_get_state(pw_ctx *ctx, double *state) {
+    if (!ctx || !state) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        ctx->ready();
+        ctx->eng->download_state_layout(ctx->job, 0, state, ctx->woff);
+    });
+}
+
+int pw_solve(pw_ctx *ctx) {
+    if (!ctx) return PW_EINVAL;
+    int st = PW_OK;
+    int rc = guarded(&ctx->err, [&] {
+        ctx->ready();
+        ctx->eng->solve();
+        st = ctx->flag_status();
+        if (st) ctx->err = "non-finite rates (coincident particles?)";
+    });
+    return rc ? rc : st;
+}
+
+int pw_get_rates(pw_ctx *ctx, double *rates) {
+    if (!ctx || !rates) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        ctx->ready();
+        ctx->eng->download_state_layout(ctx->job, 1, rates, ctx->woff);
+    });
+}
+
+int pw_get_wake(pw_ctx *ctx, size_t wake, double *pos, double *vort, double *vel,
+                double *dvort) {
+    if (!ctx) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        if (wake >= ctx->wake_n.size()) throw Error(PW_EINVAL, "no such wake");
+        ctx->ready();
+        std::vector<double> tmp(3 * ctx->total());
+        double *outs[4] = {pos, vort, vel, dvort};
+        const size_t o = 3 * (size_t)ctx->woff[wake], m = 3 * ctx->wake_n[wake];
+        for (int k = 0; k < 4; ++k) {
+            if (!outs[k]) continue;
+            ctx->eng->download_aos(ctx->job, k, tmp.data());
+            std::memcpy(outs[k], tmp.data() + o, m * sizeof(double));
+        }
+    });
+}
+
+int pw_rk4_steps(pw_ctx *ctx, double dt, size_t nsteps) {
+    if (!ctx) return PW_EINVAL;
+    int st = PW_OK;
+    int rc = guarded(&ctx->err, [&] {
+        if (!std::isfinite(dt)) throw Error(PW_EINVAL, "non-finite dt");
+        ctx->ready();
+        ctx->set_dt(dt);
+        ctx->eng->rk4(nsteps, false);
+        st = ctx->flag_status();
+        if (st) ctx->err = "non-finite rates or state";
+    });
+    return rc ? rc : st;
+}
+
+int pw_euler_steps(pw_ctx *ctx, double dt, size_t nsteps) {
+    if (!ctx) return PW_EINVAL;
+    int st = PW_OK;
+    int rc = guarded(&ctx->err, [&] {
+        if (!std::isfinite(dt)) throw Error(PW_EINVAL, "non-finite dt");
+        ctx->ready();
+        ctx->set_dt(dt);
+        ctx->eng->euler(nsteps);
+        st = ctx->flag_status();
+        if (st) ctx->err = "non-finite rates or state";
+    });
+    return rc ? rc : st;
+}
+
+int pw_diagnostics(pw_ctx *ctx, double omega[3], double impulse[3], double *ke) {
+    if (!ctx) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        ctx->ready();
+        ctx->eng->diagnostics(ctx->job, omega, impulse, ke);
+    });
+}
+
+int pw_last_timing(const pw_ctx *ctx, double *total_ms, double *pair_ms, double *pairs) {
+    if (!ctx || !ctx->eng) return PW_EINVAL;
+    if (total_ms) *total_ms = ctx->eng->last_total_ms;
+    if (pair_ms) *pair_ms = ctx->eng->last_pair_ms;
+    if (pairs) *pairs = ctx->eng->last_pairs;
+    return PW_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batch
+struct pw_batch {
+    std::string err;
+    std::unique_ptr<Engine> eng;
+    int64_t next_id = 1;
+    std::map<int64_t, int> status;
+};
+
+static JobParams params_of(const pw_job_spec *s) {
+    if (!finite_params(s->nu, s->dt, s->uinf)) throw Error(PW_EINVAL, "non-finite job parameter");
+    if (s->precision != PW_FP64 && s->precision != PW_FP32C)
+        throw Error(PW_EINVAL, "unknown precision");
+    JobParams p;
+    p.nu = s->nu;
+    p.dt = s->dt;
+    std::memcpy(p.uinf, s->uinf, sizeof p.uinf);
+    p.precision = s->precision;
+    return p;
+}
+
+extern "C" {
+
+int pw_batch_create(pw_batch **out, int device, int kernel, int block) {
+    if (!out) return PW_EINVAL;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        auto b = std::make_unique<pw_batch>();
+        b->eng.reset(new Engine(device, kernel, block ? block : 128));
+        *out = b.release();
+    });
+}
+
+int pw_batch_destroy(pw_batch *b) {
+    if (!b) return PW_EINVAL;
+    int rc = guarded(nullptr, [&] { b->eng.reset(); });
+    delete b;
+    return rc;
+}
+
+const char *pw_batch_last_error(const pw_batch *b) { return b ? b->err.c_str() : g_last_error.c_str(); }
+
+int pw_batch_add(pw_batch *b, const pw_job_spec *spec, int64_t *id) {
+    if (!b || !spec || !id) return PW_EINVAL;
+    return guarded(&b->err, [&] {
+        JobParams p = params_of(spec);
+        int64_t jid = b->next_id++;
+        b->eng->add_job(jid, spec->n, spec->pos, spec->vort, spec->radius, spec->volume, p);
+        b->status[jid] = PW_OK;
+        *id = jid;
+    });
+}
+
+int pw_batch_remove(pw_batch *b, int64_t id) {
+    if (!b) return PW_EINVAL;
+    return guarded(&b->err, [&] {
+        b->eng->remove_job(id);
+        b->status.erase(id);
+    });
+}
+
+static void batch_flags(pw_batch *b) {
+    for (auto &j : b->eng->jobs())
+        if (b->eng->take_flag(j.get())) b->status[j->id] = PW_ENONFINITE;
+}
+
+int pw_batch_step(pw_batch *b, size_t nsteps, int use_graph) {
+    if (!b) return PW_EINVAL;
+    return guarded(&b->err, [&] {
+        b->eng->rk4(nsteps, use_graph != 0);
+        batch_flags(b);
+    });
+}
+
+int pw_batch_solve(pw_batch *b) {
+    if (!b) return PW_EINVAL;
+    return guarded(&b->err, [&] {
+        b->eng->solve();
+        batch_flags(b);
+    });
+}
+
+int pw_batch_get(pw_batch *b, int64_t id, double *pos, double *vort, double *vel, double *dvort) {
+    if (!b) return PW_EINVAL;
+    return guarded(&b->err, [&] {
+        Job *j = b->eng->find(id);
+        if (!j) throw Error(PW_ENOTFOUND, "unknown job");
+        double *outs[4] = {pos, vort, vel, dvort};
+        for (int k = 0; k < 4; ++k)
+            if (outs[k]) b->eng->download_aos(j, k, outs[k]);
+    });
+}
+
+int pw_batch_status(pw_batch *b, int64_t id) {
+    if (!b) return PW_EINVAL;
+    auto it = b->status.find(id);
+    return it == b->status.end() ? PW_ENOTFOUND : it->second;
+}
+
+int pw_batch_last_timing(const pw_batch *b, double *total_ms, double *pair_ms, double *pairs) {
+    if (!b) return PW_EINVAL;
+    if (total_ms) *total_ms = b->eng->last_total_ms;
+    if (pair_ms) *pair_ms = b->eng->last_pair_ms;
+    if (pairs) *pairs = b->eng->last_pairs;
+    return PW_OK;
+}
+
+size_t pw_chunk_for(size_t n, int block) { return pw::chunk_for(n, block ? block : 128); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- serving
+namespace {
+enum JobState { JS_QUEUED = 0, JS_RUNNING = 1, JS_DONE = 2, JS_FAILED = 3, JS_CANCELLED = 4 };
+
+struct SJob {
+    int64_t id = 0;
+    JobState state = JS_QUEUED;
+    size_t n = 0, nsteps = 0, done = 0, bytes = 0;
+    JobParams par;
+    double arrival = 0.0, finished = 0.0;
+    std::vector<double> pos, vort, rad, vol;  // inputs until admitted, results when done
+};
+
+double pct(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    double r = q * (double)(v.size() - 1);
+    size_t lo = (size_t)std::floor(r), hi = (size_t)std::ceil(r);
+    return v[lo] + (v[hi] - v[lo]) * (r - (double)lo);
+}
+}  // namespace
+
+struct pw_server {
+    std::string err;
+    std::unique_ptr<Engine> eng;
+    size_t budget = 0, in_use = 0;
+    int64_t next_id = 1;
+    std::map<int64_t, SJob> jobs;
+    std::deque<int64_t> queue;  // arrival order
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    std::vector<double> step_ms, job_ms;
+    pw_server_stats st{};
+
+    double now() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+extern "C" {
+
+size_t pw_job_footprint(size_t n, int precision) { return pw::job_footprint(n, precision, 128); }
+
+int pw_server_create(pw_server **out, int device, int kernel, size_t hbm_budget_bytes) {
+    if (!out) return PW_EINVAL;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        auto s = std::make_unique<pw_server>();
+        s->eng.reset(new Engine(device, kernel, 128));
+        s->budget = hbm_budget_bytes;
+        *out = s.release();
+    });
+}
+
+int pw_server_destroy(pw_server *s) {
+    if (!s) return PW_EINVAL;
+    int rc = guarded(nullptr, [&] { s->eng.reset(); });
+    delete s;
+    return rc;
+}
+
+const char *pw_server_last_error(const pw_server *s) { return s ? s->err.c_str() : g_last_error.c_str(); }
+
+double pw_server_now_ms(const pw_server *s) { return s ? s->now() : 0.0; }
+
+int pw_server_submit(pw_server *s, const pw_job_spec *spec, size_t nsteps, double arrival_ms,
+                     int64_t *id) {
+    if (!s || !spec || !id) return PW_EINVAL;
+    return guarded(&s->err, [&] {
+        if (spec->n == 0 || !spec->pos || !spec->vort || !spec->radius || !spec->volume)
+            throw Error(PW_EINVAL, "empty job or null array");
+        if (nsteps == 0) throw Error(PW_EINVAL, "a job needs at least one step");
+        SJob j;
+        j.par = params_of(spec);
+        j.id = s->next_id++;
+        j.n = spec->n;
+        j.nsteps = nsteps;
+        j.bytes = pw::job_footprint(spec->n, spec->precision, s->eng->block());
+        if (j.bytes > s->budget) throw Error(PW_ENOMEM, "job larger than the HBM budget");
+        j.arrival = arrival_ms < 0 ? s->now() : arrival_ms;
+        j.pos.assign(spec->pos, spec->pos + 3 * spec->n);
+        j.vort.assign(spec->vort, spec->vort + 3 * spec->n);
+        j.rad.assign(spec->radius, spec->radius + spec->n);
+        j.vol.assign(spec->volume, spec->volume + spec->n);
+        *id = j.id;
+        s->queue.push_back(j.id);
+        s->jobs.emplace(j.id, std::move(j));
+        s->st.submitted++;
+    });
+}
+
+int pw_server_cancel(pw_server *s, int64_t id) {
+    if (!s) return PW_EINVAL;
+    return guarded(&s->err, [&] {
+        auto it = s->jobs.find(id);
+        if (it == s->jobs.end()) throw Error(PW_ENOTFOUND, "unknown job");
+        SJob &j = it->second;
+        if (j.state == JS_RUNNING) {
+            s->eng->remove_job(id);
+            s->in_use -= j.bytes;
+        } else if (j.state == JS_QUEUED) {
+            s->queue.erase(std::find(s->queue.begin(), s->queue.end(), id));
+        } else {
+            return;  // already finished
+        }
+        j.state = JS_CANCELLED;
+        j.pos.clear();
+        j.vort.clear();
+        s->st.cancelled++;
+    });
+}
+
+int pw_server_step(pw_server *s, int64_t *running) {
+    if (!s) return PW_EINVAL;
+    return guarded(&s->err, [&] {
+        const double t_begin = s->now();
+        // admission: queued jobs in arrival order that fit the budget
+        for (auto it = s->queue.begin(); it != s->queue.end();) {
+            SJob &j = s->jobs[*it];
+            if (s->in_use + j.bytes > s->budget) {
+                ++it;
+                continue;
+            }
+            s->eng->add_job(j.id, j.n, j.pos.data(), j.vort.data(), j.rad.data(), j.vol.data(),
+                            j.par);
+            s->in_use += j.bytes;
+            j.state = JS_RUNNING;
+            j.pos.clear();
+            j.vort.clear();
+            s->st.admitted++;
+            it = s->queue.erase(it);
+        }
+        // one batched RK4 step over every resident job (captured graph)
+        if (!s->eng->jobs().empty()) {
+            s->eng->rk4(1, true);
+            s->st.steps++;
+            s->st.pairs += s->eng->last_pairs;
+            s->st.busy_ms += s->eng->last_total_ms;
+        }
+        // retirement
+        std::vector<int64_t> ids;
+        for (auto &jp : s->eng->jobs()) ids.push_back(jp->id);
+        for (int64_t id : ids) {
+            SJob &j = s->jobs[id];
+            Job *dj = s->eng->find(id);
+            bool bad = s->eng->take_flag(dj);
+            j.done++;
+            if (bad || j.done >= j.nsteps) {
+                if (!bad) {
+                    j.pos.resize(3 * j.n);
+                    j.vort.resize(3 * j.n);
+                    s->eng->download_aos(dj, 0, j.pos.data());
+                    s->eng->download_aos(dj, 1, j.vort.data());
+                }
+                s->eng->remove_job(id);
+                s->in_use -= j.bytes;
+                j.state = bad ? JS_FAILED : JS_DONE;
+                j.finished = s->now();
+                if (bad)
+                    s->st.failed++;
+                else {
+                    s->st.finished++;
+                    s->job_ms.push_back(j.finished - j.arrival);
+                }
+            }
+        }
+        if (!ids.empty()) s->step_ms.push_back(s->now() - t_begin);
+        if (running) *running = (int64_t)s->eng->jobs().size();
+    });
+}
+
+int pw_server_job_state(pw_server *s, int64_t id, int64_t *steps_done) {
+    if (!s) return -1;
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return -1;
+    if (steps_done) *steps_done = (int64_t)it->second.done;
+    return (int)it->second.state;
+}
+
+int pw_server_fetch(pw_server *s, int64_t id, double *pos, double *vort) {
+    if (!s) return PW_EINVAL;
+    return guarded(&s->err, [&] {
+        auto it = s->jobs.find(id);
+        if (it == s->jobs.end()) throw Error(PW_ENOTFOUND, "unknown job");
+        SJob &j = it->second;
+        if (j.state == JS_FAILED) throw Error(PW_ENONFINITE, "job failed with non-finite values");
+        if (j.state != JS_DONE) throw Error(PW_EINVAL, "job not finished");
+        if (pos) std::memcpy(pos, j.pos.data(), j.pos.size() * sizeof(double));
+        if (vort) std::memcpy(vort, j.vort.data(), j.vort.size() * sizeof(double));
+        s->jobs.erase(it);
+    });
+}
+
+int pw_server_get_stats(pw_server *s, pw_server_stats *st) {
+    if (!s || !st) return PW_EINVAL;
+    *st = s->st;
+    st->queued = (int64_t)s->queue.size();
+    st->running = (int64_t)s->eng->jobs().size();
+    st->step_p50_ms = pct(s->step_ms, 0.50);
+    st->step_p99_ms = pct(s->step_ms, 0.99);
+    st->job_p50_ms = pct(s->job_ms, 0.50);
+    st->job_p99_ms = pct(s->job_ms, 0.99);
+    st->hbm_in_use = s->in_use;
+    st->hbm_budget = s->budget;
+    return PW_OK;
+}
+
+}  // extern "C"

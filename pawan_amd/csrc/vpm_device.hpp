@@ -1,0 +1,589 @@
+// vpm_device.hpp -- gfx950 device code for the vortex-particle interaction path.
+//
+// One ordered pair (target i, source j != i) contributes, with d = x_i - x_j,
+// rho = |d|, sigma^2 = (s_i^2 + s_j^2)/2 and t = (rho/sigma)^2 (reference:
+// src/interaction/interaction_utils.h:429-481 with KERNEL, highorder.h:48-60):
+//   u_i   += q (a_j x d)                                   VELOCITY  :52-58
+//   da_i  += q (a_i x a_j) + F (d.(a_i x a_j)) d           VORSTRETCH :69-100
+//          + (nu Z/(sigma^2/2)) (v_i a_j - v_j a_i)        DIFFUSION :253-286
+// The kernel is evaluated in dimensionless form: with sigma^3 q = Q(t),
+// sigma^5 F = F'(t), sigma^3 Z = Zs(t),
+//   HIGHORDER: Zs = (1.875/pi) (t+1)^-7.5
+//              Q  = [(0.25/pi)(t+1.5)(t+1)^-2 t^-1/2 - Zs] / t
+//   GAUSSIAN : Zs = (2 pi)^-1.5 exp(-t/2)
+//              Q  = [(0.25/pi) erf(sqrt(t/2)) t^-1/2 - Zs] / t
+//   both     : F' = (Zs - 3Q) / t
+// and the per-target sums are regrouped so that each pair costs as little as
+// possible:
+//   sum q (a_i x a_j)           = a_i x sum q a_j
+//   d.(a_i x a_j)               = a_i.(a_j x d)      (reuses the velocity cross)
+//   sum k (v_i a_j - v_j a_i)   = v_i sum k a_j - a_i sum k v_j
+// Each work item (target tile x source chunk) writes its NCOMP partial sums to
+// its own slot; a second kernel adds the chunks in a fixed order and applies
+// the epilogue, so results never depend on scheduling.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vpm_types.hpp"
+
+namespace pw {
+
+// ------------------------------------------------------------------ math
+// Reciprocal square root: hardware estimate (about 2^-24 relative on gfx950)
+// refined by one Newton-Raphson step r <- r (1 + (1 - t r^2)/2), which
+// squares the relative error (about 2^-47 after one step).
+__device__ __forceinline__ double rsq_nr(double t) {
+    double r = __builtin_amdgcn_rsq(t);
+    double e = __builtin_fma(-t, r * r, 1.0);
+    return __builtin_fma(r * 0.5, e, r);
+}
+
+template <int KIND>
+__device__ __forceinline__ void kernel64(double t, double rt, double &Q, double &Fp, double &Zs) {
+    const double it = rt * rt;
+    if (KIND == KIND_HIGHORDER) {
+        const double w = rsq_nr(t + 1.0);  // (t+1)^-1/2
+        const double iu = w * w;
+        const double iu2 = iu * iu;
+        const double iu4 = iu2 * iu2;
+        Zs = K_CZ * (iu4 * iu2 * iu * w);  // (t+1)^-7.5
+        const double phir = __builtin_fma(t, K_CPHI, 1.5 * K_CPHI) * (iu2 * rt);
+        Q = (phir - Zs) * it;
+    } else {
+        Zs = K_CG * exp(-0.5 * t);
+        const double phir = K_CPHI * erf(K_SQRT1_2 * (t * rt)) * rt;
+        Q = (phir - Zs) * it;
+    }
+    Fp = __builtin_fma(-3.0, Q, Zs) * it;
+}
+
+template <int KIND>
+__device__ __forceinline__ void kernel32(float t, float rt, float &Q, float &Fp, float &Zs) {
+    const float it = rt * rt;
+    if (KIND == KIND_HIGHORDER) {
+        const float w = __builtin_amdgcn_rsqf(t + 1.0f);
+        const float iu = w * w;
+        const float iu2 = iu * iu;
+        const float iu4 = iu2 * iu2;
+        Zs = (float)K_CZ * (iu4 * iu2 * iu * w);
+        const float phir = __builtin_fmaf(t, (float)K_CPHI, (float)(1.5 * K_CPHI)) * (iu2 * rt);
+        Q = (phir - Zs) * it;
+    } else {
+        Zs = (float)K_CG * __expf(-0.5f * t);
+        const float phir = (float)K_CPHI * erff((float)K_SQRT1_2 * (t * rt)) * rt;
+        Q = (phir - Zs) * it;
+    }
+    Fp = __builtin_fmaf(-3.0f, Q, Zs) * it;
+}
+
+struct Acc64 {
+    double v[NCOMP];
+};
+
+// One pair in binary64.  UNIFORM: every particle of the job has the same
+// core radius, so sigma^-3, sigma^-5 are applied once in the epilogue.
+template <int KIND, bool UNIFORM>
+__device__ __forceinline__ void pair64(double xi, double yi, double zi, double aix, double aiy,
+                                       double aiz, double hi, const Rec64 &s, double inv_s2,
+                                       Acc64 &a) {
+    const double dx = xi - s.x, dy = yi - s.y, dz = zi - s.z;
+    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+    double Q, Fp, Zs;
+    if (UNIFORM) {
+        const double t = r2 * inv_s2;
+        kernel64<KIND>(t, rsq_nr(t), Q, Fp, Zs);
+    } else {
+        const double isg = rsq_nr(hi + s.h);  // 1/sigma
+        const double is2 = isg * isg;
+        const double t = r2 * is2;
+        kernel64<KIND>(t, rsq_nr(t), Q, Fp, Zs);
+        const double s3 = isg * is2, s5 = s3 * is2;
+        Q *= s3;
+        Fp *= s5;
+        Zs *= s5;
+    }
+    // c = a_j x d
+    const double cx = __builtin_fma(s.ay, dz, -s.az * dy);
+    const double cy = __builtin_fma(s.az, dx, -s.ax * dz);
+    const double cz = __builtin_fma(s.ax, dy, -s.ay * dx);
+    a.v[0] = __builtin_fma(Q, cx, a.v[0]);
+    a.v[1] = __builtin_fma(Q, cy, a.v[1]);
+    a.v[2] = __builtin_fma(Q, cz, a.v[2]);
+    a.v[3] = __builtin_fma(Q, s.ax, a.v[3]);
+    a.v[4] = __builtin_fma(Q, s.ay, a.v[4]);
+    a.v[5] = __builtin_fma(Q, s.az, a.v[5]);
+    const double fd = Fp * __builtin_fma(aiz, cz, __builtin_fma(aiy, cy, aix * cx));
+    a.v[6] = __builtin_fma(fd, dx, a.v[6]);
+    a.v[7] = __builtin_fma(fd, dy, a.v[7]);
+    a.v[8] = __builtin_fma(fd, dz, a.v[8]);
+    a.v[9] = __builtin_fma(Zs, s.ax, a.v[9]);
+    a.v[10] = __builtin_fma(Zs, s.ay, a.v[10]);
+    a.v[11] = __builtin_fma(Zs, s.az, a.v[11]);
+    a.v[12] = __builtin_fma(Zs, s.v, a.v[12]);
+}
+
+struct Acc32 {
+    float v[NCOMP];
+};
+
+template <int KIND, bool UNIFORM>
+__device__ __forceinline__ void pair32(double xi, double yi, double zi, float aix, float aiy,
+                                       float aiz, float hi, const Rec32 &s, float inv_s2,
+                                       Acc32 &a) {
+    const float dx = (float)(xi - s.x), dy = (float)(yi - s.y), dz = (float)(zi - s.z);
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    float Q, Fp, Zs;
+    if (UNIFORM) {
+        const float t = r2 * inv_s2;
+        kernel32<KIND>(t, __builtin_amdgcn_rsqf(t), Q, Fp, Zs);
+    } else {
+        const float isg = __builtin_amdgcn_rsqf(hi + (float)s.h);
+        const float is2 = isg * isg;
+        const float t = r2 * is2;
+        kernel32<KIND>(t, __builtin_amdgcn_rsqf(t), Q, Fp, Zs);
+        const float s3 = isg * is2, s5 = s3 * is2;
+        Q *= s3;
+        Fp *= s5;
+        Zs *= s5;
+    }
+    const float cx = __builtin_fmaf(s.ay, dz, -s.az * dy);
+    const float cy = __builtin_fmaf(s.az, dx, -s.ax * dz);
+    const float cz = __builtin_fmaf(s.ax, dy, -s.ay * dx);
+    a.v[0] = __builtin_fmaf(Q, cx, a.v[0]);
+    a.v[1] = __builtin_fmaf(Q, cy, a.v[1]);
+    a.v[2] = __builtin_fmaf(Q, cz, a.v[2]);
+    a.v[3] = __builtin_fmaf(Q, s.ax, a.v[3]);
+    a.v[4] = __builtin_fmaf(Q, s.ay, a.v[4]);
+    a.v[5] = __builtin_fmaf(Q, s.az, a.v[5]);
+    const float fd = Fp * __builtin_fmaf(aiz, cz, __builtin_fmaf(aiy, cy, aix * cx));
+    a.v[6] = __builtin_fmaf(fd, dx, a.v[6]);
+    a.v[7] = __builtin_fmaf(fd, dy, a.v[7]);
+    a.v[8] = __builtin_fmaf(fd, dz, a.v[8]);
+    a.v[9] = __builtin_fmaf(Zs, s.ax, a.v[9]);
+    a.v[10] = __builtin_fmaf(Zs, s.ay, a.v[10]);
+    a.v[11] = __builtin_fmaf(Zs, s.az, a.v[11]);
+    a.v[12] = __builtin_fmaf(Zs, s.v, a.v[12]);
+}
+
+// Knuth's TwoSum: s + e == a + b exactly (round-to-nearest, no reassociation).
+__device__ __forceinline__ void two_sum_acc(float &s, float &c, float b) {
+    const float a = s;
+    const float t = a + b;
+    const float bb = t - a;
+    const float e = (a - (t - bb)) + (b - bb);
+    s = t;
+    c += e;
+}
+
+// Take the next work item from the queue; every thread of the block gets the
+// same value, so the exit below is uniform.
+__device__ __forceinline__ int next_item(int *counter, int *slot) {
+    __syncthreads();
+    if (threadIdx.x == 0) *slot = atomicAdd(counter, 1);
+    __syncthreads();
+    return *slot;
+}
+
+// ------------------------------------------------------------- pair kernels
+// Persistent: a fixed grid takes (job, target tile, source chunk) items from
+// an atomic queue until it is empty.  Thread = one target; the chunk's
+// sources are staged through LDS, ST records at a time, with coalesced
+// 16-byte loads, and read back as wave-uniform broadcasts.
+template <int KIND, int TB, int ST>
+__global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ jobs,
+                                                    const Item *__restrict__ items,
+                                                    const int *__restrict__ nitems_p,
+                                                    int *__restrict__ counter) {
+    __shared__ Rec64 tile[ST];
+    __shared__ int slot;
+    const int nitems = *nitems_p;
+    for (;;) {
+        const int it = next_item(counter, &slot);
+        if (it >= nitems) break;
+        const Item item = items[it];
+        const JobDev &J = jobs[item.job];
+        const Rec64 *rec = (const Rec64 *)J.rec;
+        const int n = J.n;
+        const int t0 = item.tile * TB;
+        const int i = t0 + (int)threadIdx.x;
+        const bool active = i < n;
+        const Rec64 me = rec[active ? i : n - 1];
+        const double inv_s2 = J.inv_s2;
+        const bool uniform = J.uniform != 0;
+        Acc64 acc;
+#pragma unroll
+        for (int m = 0; m < NCOMP; ++m) acc.v[m] = 0.0;
+        const int c0 = item.chunk * J.chunk_len;
+        const int c1 = min(c0 + J.chunk_len, n);
+        for (int s0 = c0; s0 < c1; s0 += ST) {
+            const int cnt = min(ST, c1 - s0);
+            __syncthreads();
+            {
+                const double2 *src = (const double2 *)(rec + s0);
+                double2 *dst = (double2 *)tile;
+                for (int k = threadIdx.x; k < cnt * 4; k += TB) dst[k] = src[k];
+            }
+            __syncthreads();
+            const bool diag = (s0 < t0 + TB) && (t0 < s0 + cnt);
+            if (!diag) {
+                if (uniform) {
+#pragma unroll 2
+                    for (int j = 0; j < cnt; ++j)
+                        pair64<KIND, true>(me.x, me.y, me.z, me.ax, me.ay, me.az, me.h, tile[j],
+                                           inv_s2, acc);
+                } else {
+#pragma unroll 2
+                    for (int j = 0; j < cnt; ++j)
+                        pair64<KIND, false>(me.x, me.y, me.z, me.ax, me.ay, me.az, me.h, tile[j],
+                                            inv_s2, acc);
+                }
+            } else {
+                // the tile holding the block's own targets: skip j == i
+                for (int j = 0; j < cnt; ++j) {
+                    if (s0 + j == i) continue;
+                    if (uniform)
+                        pair64<KIND, true>(me.x, me.y, me.z, me.ax, me.ay, me.az, me.h, tile[j],
+                                           inv_s2, acc);
+                    else
+                        pair64<KIND, false>(me.x, me.y, me.z, me.ax, me.ay, me.az, me.h,
+                                            tile[j], inv_s2, acc);
+                }
+            }
+        }
+        if (active) {
+            double *p = J.part + (size_t)item.chunk * NCOMP * J.npad + i;
+#pragma unroll
+            for (int m = 0; m < NCOMP; ++m) p[(size_t)m * J.npad] = acc.v[m];
+        }
+    }
+}
+
+// Compensated binary32 pair pass: plain binary32 sums over SB sources, folded
+// into a (sum, carry) pair per component with TwoSum; the item's result is
+// sum + carry in binary64, and chunks are added in binary64 afterwards.
+template <int KIND, int TB, int ST, int SB>
+__global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ jobs,
+                                                    const Item *__restrict__ items,
+                                                    const int *__restrict__ nitems_p,
+                                                    int *__restrict__ counter) {
+    static_assert(ST % SB == 0, "sub-block must divide the LDS tile");
+    __shared__ Rec32 tile[ST];
+    __shared__ int slot;
+    const int nitems = *nitems_p;
+    for (;;) {
+        const int it = next_item(counter, &slot);
+        if (it >= nitems) break;
+        const Item item = items[it];
+        const JobDev &J = jobs[item.job];
+        const Rec32 *rec = (const Rec32 *)J.rec;
+        const int n = J.n;
+        const int t0 = item.tile * TB;
+        const int i = t0 + (int)threadIdx.x;
+        const bool active = i < n;
+        const Rec32 me = rec[active ? i : n - 1];
+        const float inv_s2 = (float)J.inv_s2;
+        const float hi = (float)me.h;
+        const bool uniform = J.uniform != 0;
+        Acc32 sum, car;
+#pragma unroll
+        for (int m = 0; m < NCOMP; ++m) sum.v[m] = car.v[m] = 0.0f;
+        const int c0 = item.chunk * J.chunk_len;
+        const int c1 = min(c0 + J.chunk_len, n);
+        for (int s0 = c0; s0 < c1; s0 += ST) {
+            const int cnt = min(ST, c1 - s0);
+            __syncthreads();
+            {
+                const float4 *src = (const float4 *)(rec + s0);
+                float4 *dst = (float4 *)tile;
+                for (int k = threadIdx.x; k < cnt * 3; k += TB) dst[k] = src[k];
+            }
+            __syncthreads();
+            const bool diag = (s0 < t0 + TB) && (t0 < s0 + cnt);
+            for (int b0 = 0; b0 < cnt; b0 += SB) {
+                const int bn = min(SB, cnt - b0);
+                Acc32 a;
+#pragma unroll
+                for (int m = 0; m < NCOMP; ++m) a.v[m] = 0.0f;
+                if (!diag && bn == SB) {
+                    if (uniform) {
+#pragma unroll 4
+                        for (int j = 0; j < SB; ++j)
+                            pair32<KIND, true>(me.x, me.y, me.z, me.ax, me.ay, me.az, hi,
+                                               tile[b0 + j], inv_s2, a);
+                    } else {
+#pragma unroll 2
+                        for (int j = 0; j < SB; ++j)
+                            pair32<KIND, false>(me.x, me.y, me.z, me.ax, me.ay, me.az, hi,
+                                                tile[b0 + j], inv_s2, a);
+                    }
+                } else {
+                    for (int j = 0; j < bn; ++j) {
+                        if (s0 + b0 + j == i) continue;
+                        if (uniform)
+                            pair32<KIND, true>(me.x, me.y, me.z, me.ax, me.ay, me.az, hi,
+                                               tile[b0 + j], inv_s2, a);
+                        else
+                            pair32<KIND, false>(me.x, me.y, me.z, me.ax, me.ay, me.az, hi,
+                                                tile[b0 + j], inv_s2, a);
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < NCOMP; ++m) two_sum_acc(sum.v[m], car.v[m], a.v[m]);
+            }
+        }
+        if (active) {
+            double *p = J.part + (size_t)item.chunk * NCOMP * J.npad + i;
+#pragma unroll
+            for (int m = 0; m < NCOMP; ++m)
+                p[(size_t)m * J.npad] = (double)sum.v[m] + (double)car.v[m];
+        }
+    }
+}
+
+// --------------------------------------------------------- combine kernel
+// Per particle: add the chunk partials in chunk order (binary64), form the
+// rates
+//   u   = su * S[0:3] + uinf
+//   da  = sq * (a_i x S[3:6]) + sf * S[6:9] + sd * (v_i S[9:12] - a_i S[12])
+// (uniform core: su = sq = sigma^-3, sf = sigma^-5, sd = 2 nu sigma^-5;
+// otherwise the per-pair factors are already inside and su = sq = sf = 1,
+// sd = 2 nu), then apply the RK4 / Euler stage combination of rk4.cpp:21-66
+// with unfused multiplies and adds in the reference's order, and write the
+// next stage's record.  Block 0 also re-arms the pair kernels' queues.
+template <typename Rec>
+__device__ __forceinline__ void store_rec(Rec *r, int i, const double *xs);
+template <>
+__device__ __forceinline__ void store_rec<Rec64>(Rec64 *r, int i, const double *xs) {
+    r[i].x = xs[0];
+    r[i].y = xs[1];
+    r[i].z = xs[2];
+    r[i].ax = xs[3];
+    r[i].ay = xs[4];
+    r[i].az = xs[5];
+}
+template <>
+__device__ __forceinline__ void store_rec<Rec32>(Rec32 *r, int i, const double *xs) {
+    r[i].x = xs[0];
+    r[i].y = xs[1];
+    r[i].z = xs[2];
+    r[i].ax = (float)xs[3];
+    r[i].ay = (float)xs[4];
+    r[i].az = (float)xs[5];
+}
+
+__global__ __launch_bounds__(256) void combine_kernel(const JobDev *__restrict__ jobs,
+                                                      const CItem *__restrict__ citems,
+                                                      const int *__restrict__ ncitems_p,
+                                                      int *__restrict__ counters, int mode) {
+#pragma clang fp contract(off)
+    const int nci = *ncitems_p;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[0] = 0;
+        counters[1] = 0;
+    }
+    for (int ci = blockIdx.x; ci < nci; ci += gridDim.x) {
+        const CItem c = citems[ci];
+        const JobDev &J = jobs[c.job];
+        const int i = c.tile * 256 + (int)threadIdx.x;
+        if (i >= J.n) continue;
+        const size_t np = (size_t)J.npad;
+        double S[NCOMP];
+#pragma unroll
+        for (int m = 0; m < NCOMP; ++m) S[m] = 0.0;
+        for (int ch = 0; ch < J.nchunk; ++ch) {
+            const double *p = J.part + (size_t)ch * NCOMP * np + i;
+#pragma unroll
+            for (int m = 0; m < NCOMP; ++m) S[m] += p[m * np];
+        }
+        double xs[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) xs[m] = J.xs[m * np + i];
+        const double vi = J.vol[i];
+        double k[6];
+        k[0] = J.su * S[0] + J.uinf[0];
+        k[1] = J.su * S[1] + J.uinf[1];
+        k[2] = J.su * S[2] + J.uinf[2];
+        const double ax = xs[3], ay = xs[4], az = xs[5];
+        const double st0 = ay * S[5] - az * S[4];
+        const double st1 = az * S[3] - ax * S[5];
+        const double st2 = ax * S[4] - ay * S[3];
+        k[3] = J.sq * st0 + J.sf * S[6] + J.sd * (vi * S[9] - ax * S[12]);
+        k[4] = J.sq * st1 + J.sf * S[7] + J.sd * (vi * S[10] - ay * S[12]);
+        k[5] = J.sq * st2 + J.sf * S[8] + J.sd * (vi * S[11] - az * S[12]);
+        bool finite = true;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            J.rate[m * np + i] = k[m];
+            finite = finite && __builtin_isfinite(k[m]);
+        }
+        if (!finite) atomicOr(J.flag, 1);
+        if (mode == MODE_SOLVE) continue;
+        const double dt = J.dt;
+        double nx[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            const size_t o = m * np + i;
+            const double x0 = J.x0[o];
+            if (mode == MODE_RK4_1) {
+                J.k1[o] = k[m];
+                nx[m] = k[m] * (0.5 * dt) + x0;
+            } else if (mode == MODE_RK4_2) {
+                J.k23[o] = k[m];
+                nx[m] = k[m] * (0.5 * dt) + x0;
+            } else if (mode == MODE_RK4_3) {
+                J.k23[o] = J.k23[o] + k[m];
+                nx[m] = k[m] * dt + x0;
+            } else if (mode == MODE_RK4_4) {
+                const double a = (J.k1[o] + k[m]) * (dt / 6.);
+                const double b = J.k23[o] * (dt / 3.);
+                nx[m] = x0 + (a + b);
+                J.x0[o] = nx[m];
+            } else {  // MODE_EULER
+                nx[m] = x0 + k[m] * dt;
+                J.x0[o] = nx[m];
+            }
+            J.xs[o] = nx[m];
+            finite = finite && __builtin_isfinite(nx[m]);
+        }
+        if (!finite) atomicOr(J.flag, 1);
+        if (J.prec == PREC_FP64)
+            store_rec<Rec64>((Rec64 *)J.rec, i, nx);
+        else
+            store_rec<Rec32>((Rec32 *)J.rec, i, nx);
+    }
+}
+
+// Write the records of the current stage state xs (after a state upload).
+__global__ __launch_bounds__(256) void refresh_rec_kernel(JobDev J) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= J.n) return;
+    const size_t np = (size_t)J.npad;
+    double xs[6];
+    for (int m = 0; m < 6; ++m) xs[m] = J.xs[m * np + i];
+    if (J.prec == PREC_FP64)
+        store_rec<Rec64>((Rec64 *)J.rec, i, xs);
+    else
+        store_rec<Rec32>((Rec32 *)J.rec, i, xs);
+}
+
+// Copy x0 -> xs (a new step or a new state starts from x0).
+__global__ __launch_bounds__(256) void copy_soa_kernel(const double *__restrict__ src,
+                                                       double *__restrict__ dst, size_t count) {
+    for (size_t k = blockIdx.x * 256 + threadIdx.x; k < count; k += (size_t)gridDim.x * 256)
+        dst[k] = src[k];
+}
+
+// ------------------------------------------------ state layout conversion
+// Reference state layout (wake.cpp:87-120): for each wake w, starting at
+// 6*off[w]: positions 3*n_w (row-major N x 3) then vorticities 3*n_w.
+// SoA: [6][npad].  dir = 0: state -> SoA, dir = 1: SoA -> state.
+__global__ __launch_bounds__(256) void state_layout_kernel(double *__restrict__ state,
+                                                           double *__restrict__ soa, int npad,
+                                                           const int64_t *__restrict__ woff,
+                                                           int nwake, int dir) {
+    const int64_t ntot = woff[nwake];
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < ntot; i += (int64_t)gridDim.x * 256) {
+        int w = 0;
+        while (w + 1 < nwake && i >= woff[w + 1]) ++w;
+        const int64_t k = i - woff[w];
+        const int64_t nw = woff[w + 1] - woff[w];
+        const int64_t base = 6 * woff[w];
+        for (int c = 0; c < 3; ++c) {
+            const int64_t ip = base + 3 * k + c, iv = base + 3 * nw + 3 * k + c;
+            if (dir == 0) {
+                soa[(size_t)c * npad + i] = state[ip];
+                soa[(size_t)(3 + c) * npad + i] = state[iv];
+            } else {
+                state[ip] = soa[(size_t)c * npad + i];
+                state[iv] = soa[(size_t)(3 + c) * npad + i];
+            }
+        }
+    }
+}
+
+// N x 3 row-major <-> SoA component block ([3][npad]).
+__global__ __launch_bounds__(256) void aos_soa_kernel(double *__restrict__ aos,
+                                                      double *__restrict__ soa, int n, int npad,
+                                                      int dir) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        for (int c = 0; c < 3; ++c) {
+            if (dir == 0)
+                soa[(size_t)c * npad + i] = aos[3 * (size_t)i + c];
+            else
+                aos[3 * (size_t)i + c] = soa[(size_t)c * npad + i];
+        }
+}
+
+// ------------------------------------------------------------- diagnostics
+// Per 256-particle tile: sum a (3) and sum x x a (3), written to part6[tile].
+__global__ __launch_bounds__(256) void linear_diag_kernel(const double *__restrict__ xs, int n,
+                                                          int npad, double *__restrict__ out) {
+    __shared__ double red[6][256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        const double x = xs[i], y = xs[npad + i], z = xs[2 * (size_t)npad + i];
+        const double ax = xs[3 * (size_t)npad + i], ay = xs[4 * (size_t)npad + i],
+                     az = xs[5 * (size_t)npad + i];
+        v[0] = ax;
+        v[1] = ay;
+        v[2] = az;
+        v[3] = y * az - z * ay;
+        v[4] = z * ax - x * az;
+        v[5] = x * ay - y * ax;
+    }
+    for (int m = 0; m < 6; ++m) red[m][threadIdx.x] = v[m];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int m = 0; m < 6; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Kinetic energy pair sums (interaction_utils.h:197-240):
+// e_i = a_i.a_i/(8 pi s_i) + sum_{j != i} ((rho^2 + 2 sig^2)(a_i.a_j) + (d.a_i)(d.a_j))
+//                                        / (16 pi (rho^2 + sig^2)^1.5)
+// so that KE = 2 sum_i e_i.  Grid: (target tiles, source chunks); out is
+// [nchunk][npad], chunk 0 carries the self term.
+__global__ __launch_bounds__(256) void ke_pair_kernel(const double *__restrict__ xs,
+                                                      const double *__restrict__ rad, int n,
+                                                      int npad, int chunk_len,
+                                                      double *__restrict__ out) {
+    __shared__ double sh[7][256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int ii = i < n ? i : n - 1;
+    const double x = xs[ii], y = xs[npad + ii], z = xs[2 * (size_t)npad + ii];
+    const double ax = xs[3 * (size_t)npad + ii], ay = xs[4 * (size_t)npad + ii],
+                 az = xs[5 * (size_t)npad + ii];
+    const double hs = 0.5 * rad[ii] * rad[ii];
+    const int c0 = blockIdx.y * chunk_len, c1 = min(c0 + chunk_len, n);
+    double e = 0.0;
+    if (blockIdx.y == 0) e = (ax * ax + ay * ay + az * az) / (8.0 * K_PI * rad[ii]);
+    for (int s0 = c0; s0 < c1; s0 += 256) {
+        const int cnt = min(256, c1 - s0);
+        __syncthreads();
+        const int j = s0 + (int)threadIdx.x;
+        if ((int)threadIdx.x < cnt) {
+            for (int m = 0; m < 6; ++m) sh[m][threadIdx.x] = xs[(size_t)m * npad + j];
+            sh[6][threadIdx.x] = 0.5 * rad[j] * rad[j];
+        }
+        __syncthreads();
+        for (int k = 0; k < cnt; ++k) {
+            if (s0 + k == i) continue;
+            const double dx = sh[0][k] - x, dy = sh[1][k] - y, dz = sh[2][k] - z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double sg2 = hs + sh[6][k];
+            const double aa = ax * sh[3][k] + ay * sh[4][k] + az * sh[5][k];
+            const double da = dx * ax + dy * ay + dz * az;
+            const double db = dx * sh[3][k] + dy * sh[4][k] + dz * sh[5][k];
+            const double w = rsq_nr(r2 + sg2);
+            e += ((r2 + 2.0 * sg2) * aa + da * db) * (w * w * w) * (1.0 / (16.0 * K_PI));
+        }
+    }
+    if (i < n) out[(size_t)blockIdx.y * npad + i] = e;
+}
+
+}  // namespace pw

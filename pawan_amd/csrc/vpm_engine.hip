@@ -1,0 +1,617 @@
+// vpm_engine.hip -- host engine of the gfx950 interaction path.
+#include "vpm_engine.hpp"
+#include "vpm_device.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "../../include/pawan_hip.h"
+
+namespace pw {
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+            throw Error(PW_ENOMEM, std::string(what) + ": " + hipGetErrorString(e));
+        throw Error(PW_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+DevBuf::DevBuf(size_t b) { reset(b); }
+DevBuf::~DevBuf() {
+    if (p) (void)hipFree(p);
+}
+void DevBuf::reset(size_t b) {
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    if (b == 0) return;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        throw Error(PW_ENOMEM, "device allocation of " + std::to_string(b) + " bytes failed: " +
+                                   hipGetErrorString(e));
+    }
+    bytes = b;
+}
+
+HostBuf::~HostBuf() {
+    if (p) (void)hipHostFree(p);
+}
+void HostBuf::ensure(size_t b) {
+    if (b <= bytes) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipHostMalloc(&p, b, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        throw Error(PW_ENOMEM, "pinned host allocation failed");
+    }
+    bytes = b;
+}
+
+static size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+
+size_t chunk_for(size_t n, int TB) {
+    if (n == 0) return (size_t)TB;
+    size_t ntile = (n + TB - 1) / TB;
+    size_t nchunk = std::min<size_t>(16, ntile);
+    return (size_t)TB * ((ntile + nchunk - 1) / nchunk);
+}
+
+static size_t rec_bytes(int precision) {
+    return precision == PREC_FP64 ? sizeof(Rec64) : sizeof(Rec32);
+}
+
+size_t job_footprint(size_t n, int precision, int TB) {
+    size_t npad = round_up(std::max<size_t>(n, 1), 256);
+    size_t L = chunk_for(n, TB);
+    size_t nchunk = (n + L - 1) / L;
+    return npad * (rec_bytes(precision) + 5 * 6 * sizeof(double) + 2 * sizeof(double) +
+                   nchunk * NCOMP * sizeof(double)) +
+           256;
+}
+
+JobDev Job::dev() const {
+    JobDev d;
+    std::memset(&d, 0, sizeof d);
+    double *s = soa.as<double>();
+    const size_t blk = 6 * npad;
+    d.rec = rec.p;
+    d.x0 = s;
+    d.xs = s + blk;
+    d.k1 = s + 2 * blk;
+    d.k23 = s + 3 * blk;
+    d.rate = s + 4 * blk;
+    d.part = part.as<double>();
+    d.vol = vol.as<double>();
+    d.flag = flag.as<int>();
+    d.n = (int)n;
+    d.npad = (int)npad;
+    d.nchunk = nchunk;
+    d.chunk_len = chunk_len;
+    d.prec = par.precision;
+    d.uniform = uniform ? 1 : 0;
+    d.dt = par.dt;
+    d.nu = par.nu;
+    for (int c = 0; c < 3; ++c) d.uinf[c] = par.uinf[c];
+    if (uniform) {
+        // sigma^2 = (s^2 + s^2)/2 = s^2
+        long double s2 = (long double)sigma * sigma;
+        long double sg = sqrtl(s2);
+        long double s3 = s2 * sg, s5 = s3 * s2;
+        d.inv_s2 = (double)(1.0L / s2);
+        d.su = (double)(1.0L / s3);
+        d.sq = d.su;
+        d.sf = (double)(1.0L / s5);
+        d.sd = (double)(2.0L * par.nu / s5);
+    } else {
+        d.inv_s2 = 0.0;
+        d.su = d.sq = d.sf = 1.0;
+        d.sd = 2.0 * par.nu;
+    }
+    return d;
+}
+
+// ------------------------------------------------------------ dispatch
+typedef void (*PairFn)(const JobDev *, const Item *, const int *, int *);
+
+template <int KIND>
+static PairFn pick64(int tb) {
+    switch (tb) {
+    case 64: return pair64_kernel<KIND, 64, 64>;
+    case 128: return pair64_kernel<KIND, 128, 128>;
+    default: return pair64_kernel<KIND, 256, 256>;
+    }
+}
+template <int KIND>
+static PairFn pick32(int tb) {
+    switch (tb) {
+    case 64: return pair32_kernel<KIND, 64, 64, 32>;
+    case 128: return pair32_kernel<KIND, 128, 128, 32>;
+    default: return pair32_kernel<KIND, 256, 256, 32>;
+    }
+}
+static PairFn pair_fn(int prec, int kind, int tb) {
+    if (prec == PREC_FP64)
+        return kind == KIND_GAUSSIAN ? pick64<KIND_GAUSSIAN>(tb) : pick64<KIND_HIGHORDER>(tb);
+    return kind == KIND_GAUSSIAN ? pick32<KIND_GAUSSIAN>(tb) : pick32<KIND_HIGHORDER>(tb);
+}
+
+Engine::Engine(int device, int kind, int block) : device_(device), kind_(kind), tb_(block) {
+    if (tb_ != 64 && tb_ != 128 && tb_ != 256) throw Error(PW_EINVAL, "block must be 64, 128 or 256");
+    if (kind_ != KIND_HIGHORDER && kind_ != KIND_GAUSSIAN) throw Error(PW_EINVAL, "unknown kernel");
+    int ndev = 0;
+    PW_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) throw Error(PW_EINVAL, "no such HIP device");
+    PW_HIP(hipSetDevice(device_));
+    PW_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    PW_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device_));
+    int b64 = 0, b32 = 0;
+    PW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &b64, (const void *)pair_fn(PREC_FP64, kind_, tb_), tb_, 0));
+    PW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &b32, (const void *)pair_fn(PREC_FP32C, kind_, tb_), tb_, 0));
+    grid64_ = std::max(1, b64) * cus_;
+    grid32_ = std::max(1, b32) * cus_;
+    gridc_ = 4 * cus_;
+    counters_.reset(2 * sizeof(int));
+    PW_HIP(hipMemsetAsync(counters_.p, 0, 2 * sizeof(int), stream_));
+    PW_HIP(hipEventCreate(&ev_begin_));
+    PW_HIP(hipEventCreate(&ev_end_));
+    PW_HIP(hipStreamSynchronize(stream_));
+}
+
+Engine::~Engine() {
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    for (auto e : ev_pool_) (void)hipEventDestroy(e);
+    if (ev_begin_) (void)hipEventDestroy(ev_begin_);
+    if (ev_end_) (void)hipEventDestroy(ev_end_);
+    jobs_.clear();
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::sync() { PW_HIP(hipStreamSynchronize(stream_)); }
+
+Job *Engine::find(int64_t id) {
+    for (auto &j : jobs_)
+        if (j->id == id) return j.get();
+    return nullptr;
+}
+
+Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort,
+                     const double *radius, const double *volume, const JobParams &par,
+                     size_t chunk_override) {
+    if (n == 0 || !pos || !vort || !radius || !volume) throw Error(PW_EINVAL, "empty job or null array");
+    if (n > (size_t)INT32_MAX / 2) throw Error(PW_EINVAL, "too many particles");
+    if (par.precision != PREC_FP64 && par.precision != PREC_FP32C)
+        throw Error(PW_EINVAL, "unknown precision");
+    PW_HIP(hipSetDevice(device_));
+    auto j = std::make_unique<Job>();
+    j->id = id;
+    j->n = n;
+    j->npad = round_up(n, 256);
+    j->par = par;
+    size_t L = chunk_override ? chunk_override : chunk_for(n, tb_);
+    if (L % (size_t)tb_ != 0) throw Error(PW_EINVAL, "chunk must be a multiple of the block size");
+    j->chunk_len = (int)L;
+    j->nchunk = (int)((n + L - 1) / L);
+    j->ntile = (int)((n + tb_ - 1) / tb_);
+    j->uniform = true;
+    for (size_t i = 1; i < n; ++i)
+        if (std::memcmp(&radius[i], &radius[0], sizeof(double)) != 0) {
+            j->uniform = false;
+            break;
+        }
+    j->sigma = radius[0];
+    const size_t np = j->npad;
+    j->rec.reset(np * rec_bytes(par.precision));
+    j->soa.reset(5 * 6 * np * sizeof(double));
+    j->part.reset((size_t)j->nchunk * NCOMP * np * sizeof(double));
+    j->vol.reset(2 * np * sizeof(double));  // volume | radius
+    j->flag.reset(256);
+    j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + j->flag.bytes;
+    // host-side staging of records, state and constants
+    size_t need = std::max(np * rec_bytes(par.precision), 6 * np * sizeof(double));
+    stage_.ensure(std::max(need, 2 * np * sizeof(double)));
+    if (par.precision == PREC_FP64) {
+        Rec64 *r = stage_.as<Rec64>();
+        std::memset(r, 0, np * sizeof(Rec64));
+        for (size_t i = 0; i < n; ++i) {
+            r[i].x = pos[3 * i];
+            r[i].y = pos[3 * i + 1];
+            r[i].z = pos[3 * i + 2];
+            r[i].h = 0.5 * radius[i] * radius[i];
+            r[i].ax = vort[3 * i];
+            r[i].ay = vort[3 * i + 1];
+            r[i].az = vort[3 * i + 2];
+            r[i].v = volume[i];
+        }
+    } else {
+        Rec32 *r = stage_.as<Rec32>();
+        std::memset(r, 0, np * sizeof(Rec32));
+        for (size_t i = 0; i < n; ++i) {
+            r[i].x = pos[3 * i];
+            r[i].y = pos[3 * i + 1];
+            r[i].z = pos[3 * i + 2];
+            r[i].h = 0.5 * radius[i] * radius[i];
+            r[i].ax = (float)vort[3 * i];
+            r[i].ay = (float)vort[3 * i + 1];
+            r[i].az = (float)vort[3 * i + 2];
+            r[i].v = (float)volume[i];
+        }
+    }
+    PW_HIP(hipMemcpyAsync(j->rec.p, stage_.p, np * rec_bytes(par.precision), hipMemcpyHostToDevice,
+                          stream_));
+    PW_HIP(hipStreamSynchronize(stream_));
+    double *v = stage_.as<double>();
+    std::memset(v, 0, 2 * np * sizeof(double));
+    for (size_t i = 0; i < n; ++i) {
+        v[i] = volume[i];
+        v[np + i] = radius[i];
+    }
+    PW_HIP(hipMemcpyAsync(j->vol.p, v, 2 * np * sizeof(double), hipMemcpyHostToDevice, stream_));
+    PW_HIP(hipStreamSynchronize(stream_));
+    PW_HIP(hipMemsetAsync(j->soa.p, 0, j->soa.bytes, stream_));
+    PW_HIP(hipMemsetAsync(j->flag.p, 0, j->flag.bytes, stream_));
+    Job *raw = j.get();
+    jobs_.push_back(std::move(j));
+    plan_dirty_ = true;
+    upload_state_soa(raw, pos, vort);
+    return raw;
+}
+
+void Engine::remove_job(int64_t id) {
+    PW_HIP(hipSetDevice(device_));
+    PW_HIP(hipStreamSynchronize(stream_));
+    for (size_t k = 0; k < jobs_.size(); ++k)
+        if (jobs_[k]->id == id) {
+            jobs_.erase(jobs_.begin() + k);
+            plan_dirty_ = true;
+            return;
+        }
+    throw Error(PW_ENOTFOUND, "unknown job");
+}
+
+void Engine::set_params(Job *j, const JobParams &par) {
+    if (par.precision != j->par.precision) throw Error(PW_EINVAL, "precision of a job is fixed");
+    j->par = par;
+    plan_dirty_ = true;
+}
+
+void Engine::build_plan() {
+    PW_HIP(hipSetDevice(device_));
+    // queued kernels may still read the current tables and their host copy
+    PW_HIP(hipStreamSynchronize(stream_));
+    std::vector<JobDev> jd;
+    std::vector<Item> it64, it32;
+    std::vector<CItem> ci;
+    for (size_t k = 0; k < jobs_.size(); ++k) {
+        const Job &j = *jobs_[k];
+        jd.push_back(j.dev());
+        auto &dst = j.par.precision == PREC_FP64 ? it64 : it32;
+        for (int c = 0; c < j.nchunk; ++c)
+            for (int t = 0; t < j.ntile; ++t) dst.push_back(Item{(int)k, t, c, j.chunk_len});
+        for (int t = 0; t < (int)((j.n + 255) / 256); ++t) ci.push_back(CItem{(int)k, t});
+    }
+    // larger items first (their cost is tile x chunk length): shorter tail
+    auto bigger = [](const Item &a, const Item &b) { return a.pad > b.pad; };
+    std::stable_sort(it64.begin(), it64.end(), bigger);
+    std::stable_sort(it32.begin(), it32.end(), bigger);
+    n64_ = (int)it64.size();
+    n32_ = (int)it32.size();
+    nci_ = (int)ci.size();
+    size_t o_jobs = 0;
+    size_t o_i64 = round_up(o_jobs + std::max<size_t>(1, jd.size()) * sizeof(JobDev), 256);
+    size_t o_i32 = round_up(o_i64 + std::max<size_t>(1, it64.size()) * sizeof(Item), 256);
+    size_t o_ci = round_up(o_i32 + std::max<size_t>(1, it32.size()) * sizeof(Item), 256);
+    size_t o_cnt = round_up(o_ci + std::max<size_t>(1, ci.size()) * sizeof(CItem), 256);
+    size_t total = o_cnt + 256;
+    if (total > plan_.bytes) {
+        PW_HIP(hipStreamSynchronize(stream_));
+        plan_.reset(round_up(total * 3 / 2, 4096));
+    }
+    plan_host_.ensure(plan_.bytes);
+    char *h = plan_host_.as<char>();
+    std::memcpy(h + o_jobs, jd.data(), jd.size() * sizeof(JobDev));
+    std::memcpy(h + o_i64, it64.data(), it64.size() * sizeof(Item));
+    std::memcpy(h + o_i32, it32.data(), it32.size() * sizeof(Item));
+    std::memcpy(h + o_ci, ci.data(), ci.size() * sizeof(CItem));
+    int cnt[4] = {n64_, n32_, nci_, 0};
+    std::memcpy(h + o_cnt, cnt, sizeof cnt);
+    PW_HIP(hipMemcpyAsync(plan_.p, h, total, hipMemcpyHostToDevice, stream_));
+    char *d = plan_.as<char>();
+    d_jobs_ = (JobDev *)(d + o_jobs);
+    d_items64_ = (Item *)(d + o_i64);
+    d_items32_ = (Item *)(d + o_i32);
+    d_citems_ = (CItem *)(d + o_ci);
+    d_counts_ = (int *)(d + o_cnt);
+    plan_dirty_ = false;
+}
+
+double Engine::pairs_per_rhs() const {
+    double p = 0.0;
+    for (auto &j : jobs_) p += (double)j->n * (double)(j->n - 1);
+    return p;
+}
+
+void Engine::launch_stage(int mode, bool timed) {
+    int *cnt = counters_.as<int>();
+    const bool graph = !timed;
+    auto launch_pair = [&](int prec, int grid, const Item *items, const int *nitems, int *queue,
+                           int count) {
+        if (!graph && count == 0) return;
+        PairFn fn = pair_fn(prec, kind_, tb_);
+        hipEvent_t a = nullptr, b = nullptr;
+        if (timed) {
+            size_t k = (size_t)pair_launches * 2;
+            while (ev_pool_.size() < k + 2) {
+                hipEvent_t e;
+                PW_HIP(hipEventCreate(&e));
+                ev_pool_.push_back(e);
+            }
+            a = ev_pool_[k];
+            b = ev_pool_[k + 1];
+            PW_HIP(hipEventRecord(a, stream_));
+        }
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(tb_), 0, stream_, d_jobs_, items, nitems, queue);
+        PW_HIP(hipGetLastError());
+        if (timed) {
+            PW_HIP(hipEventRecord(b, stream_));
+            ++pair_launches;
+        }
+    };
+    launch_pair(PREC_FP64, grid64_, d_items64_, d_counts_ + 0, cnt + 0, n64_);
+    launch_pair(PREC_FP32C, grid32_, d_items32_, d_counts_ + 1, cnt + 1, n32_);
+    hipLaunchKernelGGL(combine_kernel, dim3(gridc_), dim3(256), 0, stream_, (const JobDev *)d_jobs_,
+                       (const CItem *)d_citems_, (const int *)(d_counts_ + 2), cnt, mode);
+    PW_HIP(hipGetLastError());
+}
+
+static void begin_timing(hipEvent_t e, hipStream_t s) { PW_HIP(hipEventRecord(e, s)); }
+
+void Engine::solve() {
+    PW_HIP(hipSetDevice(device_));
+    if (plan_dirty_) build_plan();
+    pair_launches = 0;
+    begin_timing(ev_begin_, stream_);
+    launch_stage(MODE_SOLVE, true);
+    PW_HIP(hipEventRecord(ev_end_, stream_));
+    PW_HIP(hipEventSynchronize(ev_end_));
+    float ms = 0.f;
+    PW_HIP(hipEventElapsedTime(&ms, ev_begin_, ev_end_));
+    last_total_ms = ms;
+    last_pair_ms = 0.0;
+    for (int64_t k = 0; k < pair_launches; ++k) {
+        PW_HIP(hipEventElapsedTime(&ms, ev_pool_[2 * k], ev_pool_[2 * k + 1]));
+        last_pair_ms += ms;
+    }
+    last_pairs = pairs_per_rhs();
+}
+
+void Engine::capture_graph() {
+    if (graph_exec_) {
+        (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+    if (graph_) {
+        (void)hipGraphDestroy(graph_);
+        graph_ = nullptr;
+    }
+    PW_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+        launch_stage(MODE_RK4_1, false);
+        launch_stage(MODE_RK4_2, false);
+        launch_stage(MODE_RK4_3, false);
+        launch_stage(MODE_RK4_4, false);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    PW_HIP(hipStreamEndCapture(stream_, &graph_));
+    PW_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+    graph_plan_ptr_ = plan_.p;
+}
+
+void Engine::rk4(size_t nsteps, bool use_graph) {
+    PW_HIP(hipSetDevice(device_));
+    if (plan_dirty_) build_plan();
+    if (jobs_.empty() || nsteps == 0) {
+        last_total_ms = last_pair_ms = last_pairs = 0.0;
+        return;
+    }
+    pair_launches = 0;
+    if (use_graph && (!graph_exec_ || graph_plan_ptr_ != plan_.p)) capture_graph();
+    begin_timing(ev_begin_, stream_);
+    for (size_t s = 0; s < nsteps; ++s) {
+        if (use_graph) {
+            PW_HIP(hipGraphLaunch(graph_exec_, stream_));
+        } else {
+            launch_stage(MODE_RK4_1, true);
+            launch_stage(MODE_RK4_2, true);
+            launch_stage(MODE_RK4_3, true);
+            launch_stage(MODE_RK4_4, true);
+        }
+    }
+    PW_HIP(hipEventRecord(ev_end_, stream_));
+    PW_HIP(hipEventSynchronize(ev_end_));
+    float ms = 0.f;
+    PW_HIP(hipEventElapsedTime(&ms, ev_begin_, ev_end_));
+    last_total_ms = ms;
+    last_pair_ms = 0.0;
+    for (int64_t k = 0; k < pair_launches; ++k) {
+        PW_HIP(hipEventElapsedTime(&ms, ev_pool_[2 * k], ev_pool_[2 * k + 1]));
+        last_pair_ms += ms;
+    }
+    last_pairs = 4.0 * (double)nsteps * pairs_per_rhs();
+}
+
+void Engine::euler(size_t nsteps) {
+    PW_HIP(hipSetDevice(device_));
+    if (plan_dirty_) build_plan();
+    pair_launches = 0;
+    begin_timing(ev_begin_, stream_);
+    for (size_t s = 0; s < nsteps; ++s) launch_stage(MODE_EULER, true);
+    PW_HIP(hipEventRecord(ev_end_, stream_));
+    PW_HIP(hipEventSynchronize(ev_end_));
+    float ms = 0.f;
+    PW_HIP(hipEventElapsedTime(&ms, ev_begin_, ev_end_));
+    last_total_ms = ms;
+    last_pair_ms = 0.0;
+    for (int64_t k = 0; k < pair_launches; ++k) {
+        PW_HIP(hipEventElapsedTime(&ms, ev_pool_[2 * k], ev_pool_[2 * k + 1]));
+        last_pair_ms += ms;
+    }
+    last_pairs = (double)nsteps * pairs_per_rhs();
+}
+
+static int grid_for(size_t n) { return (int)std::min<size_t>(4096, (n + 255) / 256 + 1); }
+
+void Engine::upload_state_soa(Job *j, const double *pos, const double *vort) {
+    PW_HIP(hipSetDevice(device_));
+    const size_t np = j->npad, n = j->n;
+    if (dstage_.bytes < 3 * n * sizeof(double)) {
+        PW_HIP(hipStreamSynchronize(stream_));
+        dstage_.reset(round_up(6 * np * sizeof(double), 4096));
+    }
+    JobDev d = j->dev();
+    const double *src[2] = {pos, vort};
+    for (int h = 0; h < 2; ++h) {
+        PW_HIP(hipMemcpyAsync(dstage_.p, src[h], 3 * n * sizeof(double), hipMemcpyHostToDevice,
+                              stream_));
+        hipLaunchKernelGGL(aos_soa_kernel, dim3(grid_for(n)), dim3(256), 0, stream_,
+                           dstage_.as<double>(), d.x0 + (size_t)3 * h * np, (int)n, (int)np, 0);
+        PW_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(copy_soa_kernel, dim3(grid_for(6 * np)), dim3(256), 0, stream_,
+                       (const double *)d.x0, d.xs, 6 * np);
+    hipLaunchKernelGGL(refresh_rec_kernel, dim3((n + 255) / 256), dim3(256), 0, stream_, d);
+    PW_HIP(hipGetLastError());
+    PW_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::download_aos(Job *j, int which, double *out) {
+    PW_HIP(hipSetDevice(device_));
+    const size_t np = j->npad, n = j->n;
+    if (dstage_.bytes < 3 * n * sizeof(double)) {
+        PW_HIP(hipStreamSynchronize(stream_));
+        dstage_.reset(round_up(6 * np * sizeof(double), 4096));
+    }
+    JobDev d = j->dev();
+    double *src = which == 0 ? d.x0 : which == 1 ? d.x0 + 3 * np : which == 2 ? d.rate : d.rate + 3 * np;
+    hipLaunchKernelGGL(aos_soa_kernel, dim3(grid_for(n)), dim3(256), 0, stream_,
+                       dstage_.as<double>(), src, (int)n, (int)np, 1);
+    PW_HIP(hipGetLastError());
+    PW_HIP(hipMemcpyAsync(out, dstage_.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    PW_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::upload_state_layout(Job *j, const double *state, const std::vector<int64_t> &woff) {
+    PW_HIP(hipSetDevice(device_));
+    const size_t np = j->npad, n = j->n;
+    const size_t wbytes = woff.size() * sizeof(int64_t);
+    const size_t need = 6 * n * sizeof(double) + 256 + wbytes;
+    if (dstage_.bytes < need) {
+        PW_HIP(hipStreamSynchronize(stream_));
+        dstage_.reset(round_up(need + 6 * np * sizeof(double), 4096));
+    }
+    char *base = dstage_.as<char>();
+    int64_t *dwoff = (int64_t *)(base + round_up(6 * n * sizeof(double), 256));
+    PW_HIP(hipMemcpyAsync(base, state, 6 * n * sizeof(double), hipMemcpyHostToDevice, stream_));
+    PW_HIP(hipMemcpyAsync(dwoff, woff.data(), wbytes, hipMemcpyHostToDevice, stream_));
+    JobDev d = j->dev();
+    hipLaunchKernelGGL(state_layout_kernel, dim3(grid_for(n)), dim3(256), 0, stream_,
+                       (double *)base, d.x0, (int)np, (const int64_t *)dwoff, (int)woff.size() - 1, 0);
+    hipLaunchKernelGGL(copy_soa_kernel, dim3(grid_for(6 * np)), dim3(256), 0, stream_,
+                       (const double *)d.x0, d.xs, 6 * np);
+    hipLaunchKernelGGL(refresh_rec_kernel, dim3((n + 255) / 256), dim3(256), 0, stream_, d);
+    PW_HIP(hipGetLastError());
+    PW_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::download_state_layout(Job *j, int which, double *state,
+                                   const std::vector<int64_t> &woff) {
+    PW_HIP(hipSetDevice(device_));
+    const size_t np = j->npad, n = j->n;
+    const size_t wbytes = woff.size() * sizeof(int64_t);
+    const size_t need = 6 * n * sizeof(double) + 256 + wbytes;
+    if (dstage_.bytes < need) {
+        PW_HIP(hipStreamSynchronize(stream_));
+        dstage_.reset(round_up(need + 6 * np * sizeof(double), 4096));
+    }
+    char *base = dstage_.as<char>();
+    int64_t *dwoff = (int64_t *)(base + round_up(6 * n * sizeof(double), 256));
+    PW_HIP(hipMemcpyAsync(dwoff, woff.data(), wbytes, hipMemcpyHostToDevice, stream_));
+    JobDev d = j->dev();
+    hipLaunchKernelGGL(state_layout_kernel, dim3(grid_for(n)), dim3(256), 0, stream_,
+                       (double *)base, which == 0 ? d.x0 : d.rate, (int)np,
+                       (const int64_t *)dwoff, (int)woff.size() - 1, 1);
+    PW_HIP(hipGetLastError());
+    PW_HIP(hipMemcpyAsync(state, base, 6 * n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    PW_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::diagnostics(Job *j, double omega[3], double impulse[3], double *ke) {
+    PW_HIP(hipSetDevice(device_));
+    const size_t np = j->npad, n = j->n;
+    JobDev d = j->dev();
+    const int nt = (int)((n + 255) / 256);
+    if (omega || impulse) {
+        DevBuf out(nt * 6 * sizeof(double));
+        hipLaunchKernelGGL(linear_diag_kernel, dim3(nt), dim3(256), 0, stream_, (const double *)d.x0,
+                           (int)n, (int)np, out.as<double>());
+        PW_HIP(hipGetLastError());
+        std::vector<double> h(nt * 6);
+        PW_HIP(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              stream_));
+        PW_HIP(hipStreamSynchronize(stream_));
+        double s[6] = {0, 0, 0, 0, 0, 0};
+        for (int t = 0; t < nt; ++t)
+            for (int m = 0; m < 6; ++m) s[m] += h[t * 6 + m];
+        if (omega)
+            for (int c = 0; c < 3; ++c) omega[c] = s[c];
+        if (impulse)
+            for (int c = 0; c < 3; ++c) impulse[c] = 0.5 * s[3 + c];
+    }
+    if (ke) {
+        const int L = (int)chunk_for(n, 256);
+        const int nc = (int)((n + L - 1) / L);
+        DevBuf out((size_t)nc * np * sizeof(double));
+        hipLaunchKernelGGL(ke_pair_kernel, dim3(nt, nc), dim3(256), 0, stream_, (const double *)d.x0,
+                           (const double *)(d.vol + np), (int)n, (int)np, L, out.as<double>());
+        PW_HIP(hipGetLastError());
+        std::vector<double> h((size_t)nc * np);
+        PW_HIP(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              stream_));
+        PW_HIP(hipStreamSynchronize(stream_));
+        double e = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            double ei = 0.0;
+            for (int c = 0; c < nc; ++c) ei += h[(size_t)c * np + i];
+            e += ei;
+        }
+        *ke = 2.0 * e;
+    }
+}
+
+bool Engine::take_flag(Job *j) {
+    PW_HIP(hipSetDevice(device_));
+    int f = 0;
+    PW_HIP(hipMemcpyAsync(&f, j->flag.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    PW_HIP(hipMemsetAsync(j->flag.p, 0, sizeof(int), stream_));
+    PW_HIP(hipStreamSynchronize(stream_));
+    return f != 0;
+}
+
+}  // namespace pw

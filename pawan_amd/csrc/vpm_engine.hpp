@@ -1,0 +1,157 @@
+// vpm_engine.hpp -- host side of the gfx950 interaction path: device-resident
+// jobs, the work-item plan, kernel dispatch, HIP-graph replay of a step, and
+// timing.  Shared by the single-simulation context, the batch and the server.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "vpm_types.hpp"
+
+namespace pw {
+
+// Error carrying a PW_E* status; converted to a return code at the C ABI.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what);
+#define PW_HIP(x) ::pw::hip_check((x), #x)
+
+// Owned device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t b);
+    ~DevBuf();
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    void reset(size_t b);  // reallocate (contents lost)
+    template <typename T>
+    T *as() const { return (T *)p; }
+};
+
+// Pinned host staging buffer.
+struct HostBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    HostBuf() = default;
+    ~HostBuf();
+    void ensure(size_t b);
+    template <typename T>
+    T *as() const { return (T *)p; }
+};
+
+struct JobParams {
+    double nu = 2.5e-3;
+    double dt = 0.0;
+    double uinf[3] = {0, 0, 0};
+    int precision = PREC_FP64;
+};
+
+// Source chunk length for a job of n particles with TB targets per tile:
+// at most 16 chunks per job, chunk a multiple of TB.  Depends on (n, TB) only,
+// so a job's sums are formed the same way alone or in any batch.
+size_t chunk_for(size_t n, int TB);
+
+// Bytes a resident job occupies on the device.
+size_t job_footprint(size_t n, int precision, int TB);
+
+// One simulation resident on the device.
+struct Job {
+    int64_t id = 0;
+    size_t n = 0, npad = 0;
+    int nchunk = 0, chunk_len = 0, ntile = 0;
+    JobParams par;
+    bool uniform = false;
+    double sigma = 0.0;
+    DevBuf rec, soa, part, vol, flag;  // soa holds x0, xs, k1, k23, rate
+    size_t bytes = 0;
+    JobDev dev() const;
+};
+
+class Engine {
+  public:
+    Engine(int device, int kind, int block);
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    int device() const { return device_; }
+    int kind() const { return kind_; }
+    int block() const { return tb_; }
+    hipStream_t stream() const { return stream_; }
+
+    // Create a job from N x 3 positions/vorticities and N radius/volume.
+    // chunk_override > 0 fixes the source chunk length (multiple of block).
+    Job *add_job(int64_t id, size_t n, const double *pos, const double *vort,
+                 const double *radius, const double *volume, const JobParams &par,
+                 size_t chunk_override = 0);
+    void remove_job(int64_t id);
+    Job *find(int64_t id);
+    const std::vector<std::unique_ptr<Job>> &jobs() const { return jobs_; }
+    void set_params(Job *j, const JobParams &par);
+
+    // Evaluate rates of every job (solve()).
+    void solve();
+    // nsteps RK4 / Euler steps of every job.  use_graph replays a captured
+    // step (RK4 only); timing of individual pair launches is only kept when
+    // use_graph is false.
+    void rk4(size_t nsteps, bool use_graph);
+    void euler(size_t nsteps);
+
+    // Host <-> device transfers (synchronous on the engine stream).
+    void upload_state_soa(Job *j, const double *pos, const double *vort);  // N x 3 each
+    void download_aos(Job *j, int which, double *out);  // which: 0 pos 1 vort 2 vel 3 dvort
+    void upload_state_layout(Job *j, const double *state, const std::vector<int64_t> &woff);
+    void download_state_layout(Job *j, int which, double *state,
+                               const std::vector<int64_t> &woff);  // which: 0 state 1 rates
+    // diagnostics of one job: sum a, 0.5 sum x x a, kinetic energy
+    void diagnostics(Job *j, double omega[3], double impulse[3], double *ke);
+    // non-finite flags, read back and cleared
+    bool take_flag(Job *j);
+
+    void sync();
+
+    // timing of the last solve/rk4/euler call
+    double last_total_ms = 0.0, last_pair_ms = 0.0, last_pairs = 0.0;
+    int64_t pair_launches = 0;
+
+  private:
+    void build_plan();
+    void launch_stage(int mode, bool timed);
+    void capture_graph();
+    double pairs_per_rhs() const;
+
+    int device_, kind_, tb_;
+    hipStream_t stream_ = nullptr;
+    int cus_ = 0;
+    int grid64_ = 0, grid32_ = 0, gridc_ = 0;
+    std::vector<std::unique_ptr<Job>> jobs_;
+    bool plan_dirty_ = true;
+    // plan tables: JobDev[njobs] | Item64[...] | Item32[...] | CItem[...] | ints
+    DevBuf plan_;
+    HostBuf plan_host_;
+    JobDev *d_jobs_ = nullptr;
+    Item *d_items64_ = nullptr, *d_items32_ = nullptr;
+    CItem *d_citems_ = nullptr;
+    int *d_counts_ = nullptr;  // [0] n64 [1] n32 [2] nci
+    DevBuf counters_;          // [0] queue64 [1] queue32
+    int n64_ = 0, n32_ = 0, nci_ = 0;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    void *graph_plan_ptr_ = nullptr;
+    std::vector<hipEvent_t> ev_pool_;
+    hipEvent_t ev_begin_ = nullptr, ev_end_ = nullptr;
+    HostBuf stage_;
+    DevBuf dstage_;
+};
+
+}  // namespace pw

@@ -1,0 +1,74 @@
+// vpm_types.hpp -- data layout shared by the gfx950 kernels and the host engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pw {
+
+constexpr int KIND_HIGHORDER = 0;
+constexpr int KIND_GAUSSIAN = 1;
+constexpr int PREC_FP64 = 0;
+constexpr int PREC_FP32C = 1;
+
+// Partial-sum components per (target, chunk):
+//   0-2 sum q (a_j x d)   3-5 sum q a_j   6-8 sum F (a_i.(a_j x d)) d
+//   9-11 sum k a_j        12 sum k v_j
+constexpr int NCOMP = 13;
+
+// Combine-kernel modes
+constexpr int MODE_SOLVE = 0;
+constexpr int MODE_RK4_1 = 1;
+constexpr int MODE_RK4_2 = 2;
+constexpr int MODE_RK4_3 = 3;
+constexpr int MODE_RK4_4 = 4;
+constexpr int MODE_EULER = 5;
+
+constexpr double K_PI = 3.14159265358979323846;
+constexpr double K_CZ = 1.875 / K_PI;   // HIGHORDER zeta(0)
+constexpr double K_CPHI = 0.25 / K_PI;  // phi prefactor
+constexpr double K_CG = 0.063493635934240969;  // (2 pi)^-1.5
+constexpr double K_SQRT1_2 = 0.70710678118654752440;
+
+// Source/target record of the binary64 path: 64 bytes, read as 4 x 16 B.
+// h = s^2 / 2, so that sigma^2 = h_i + h_j.
+struct __align__(16) Rec64 {
+    double x, y, z, h;
+    double ax, ay, az, v;
+};
+// Record of the compensated binary32 path: binary64 positions (the
+// displacement is formed in binary64 and then rounded), binary32 strengths.
+struct __align__(16) Rec32 {
+    double x, y, z, h;
+    float ax, ay, az, v;
+};
+
+// One simulation as the kernels see it.  Arrays are structure-of-arrays with
+// a padded stride npad (multiple of 256): component m of particle i is at
+// [m * npad + i].
+struct JobDev {
+    void *rec;      // Rec64[npad] or Rec32[npad]: the stage being evaluated
+    double *x0;     // [6][npad] state at the start of the step (x y z ax ay az)
+    double *xs;     // [6][npad] state of the stage being evaluated
+    double *k1;     // [6][npad] RK4 k1
+    double *k23;    // [6][npad] RK4 k2 (+ k3)
+    double *rate;   // [6][npad] last rates (u, da/dt)
+    double *part;   // [nchunk][NCOMP][npad] per-chunk partial sums
+    const double *vol;  // [npad] volumes
+    int *flag;      // set to 1 when a rate or state is non-finite
+    int n, npad, nchunk, chunk_len;
+    int prec, uniform, pad0, pad1;
+    double inv_s2;  // uniform core: 1/sigma^2
+    double su, sq, sf, sd;  // epilogue scales (see combine_kernel)
+    double dt, nu;
+    double uinf[3];
+    double pad2;
+};
+
+struct Item {
+    int job, tile, chunk, pad;
+};
+struct CItem {
+    int job, tile;
+};
+
+}  // namespace pw

@@ -1,0 +1,337 @@
+"""Python mirror of pawan's interaction / integration API over the HIP library.
+
+The classes follow the reference's C++ classes method for method (names,
+argument meaning, state layout) so that code written against pawan reads
+the same:
+
+    pawan                                    pawan_amd
+    __interaction(W) / (W1, W2)   -> Interaction(W) / Interaction(W1, W2)
+      addWake, solve, setStates, getStates, getRates, diagnose
+    __integration(t, n).integrate -> Integration(t, n).integrate
+    __rk4(t, n)                   -> RK4(t, n)
+
+A wake is any object with the reference's public particle members
+(``_position`` N x 3, ``_vorticity`` N x 3, ``_radius`` N, ``_volume`` N,
+src/wake/wake.h:33-42); ``Wake`` below is a minimal one.  Velocities and
+vorticity rates are written back to ``_velocity`` / ``_retvorcity`` on
+``getRates``/``sync_wakes``, as the reference's matrices are.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+_lib_h = None
+
+
+def lib():
+    global _lib_h
+    if _lib_h is None:
+        _lib_h = _lib.load()
+    return _lib_h
+
+
+def _check(rc: int, handle, errfn: str):
+    if rc != _lib.PW_OK:
+        msg = getattr(lib(), errfn)(handle)
+        raise _lib.PawanError(rc, msg.decode() if msg else "")
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    if shape is not None and a.shape != shape:
+        a = a.reshape(shape)
+    return a
+
+
+class Wake:
+    """Particle storage with the reference's public members (wake.h:33-42)."""
+
+    def __init__(self, pos, vort, radius, volume):
+        self._position = _f64(pos).reshape(-1, 3).copy()
+        self._numParticles = self._position.shape[0]
+        self._vorticity = _f64(vort).reshape(-1, 3).copy()
+        self._radius = _f64(radius).reshape(-1).copy()
+        self._volume = _f64(volume).reshape(-1).copy()
+        self._velocity = np.zeros_like(self._position)
+        self._retvorcity = np.zeros_like(self._position)
+        self._size = 6 * self._numParticles  # wake.cpp:12
+
+    @classmethod
+    def from_dict(cls, w):
+        return cls(w["pos"], w["vort"], w["radius"], w["volume"])
+
+
+class Interaction:
+    """GPU counterpart of pawan::__interaction (interaction.h:21-235)."""
+
+    def __init__(self, *wakes, nu: float = 2.5e-3, kernel: int = _lib.KERNEL_HIGHORDER,
+                 precision: int = _lib.FP64, uinf=(0.0, 0.0, 0.0), device: int = 0,
+                 block: int = 0, chunk: int = 0):
+        L = lib()
+        cfg = _lib.PwConfig()
+        L.pw_config_default(C.byref(cfg))
+        cfg.device, cfg.kernel, cfg.precision, cfg.nu = device, kernel, precision, nu
+        cfg.uinf[:] = list(uinf)
+        cfg.block, cfg.chunk = block, chunk
+        h = C.c_void_p()
+        rc = L.pw_create(C.byref(h), C.byref(cfg))
+        if rc != _lib.PW_OK:
+            raise _lib.PawanError(rc, L.pw_last_error(None).decode())
+        self._h = h
+        self._W = []
+        self._nu = nu
+        self._size = 0
+        for w in wakes:
+            self.addWake(w)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pw_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        _check(rc, self._h, "pw_last_error")
+
+    # interaction.cpp:35-39
+    def addWake(self, W):
+        self._ck(lib().pw_add_wake(self._h, W._numParticles, _lib.ptr(W._position),
+                                   _lib.ptr(W._vorticity), _lib.ptr(W._radius),
+                                   _lib.ptr(W._volume)))
+        self._W.append(W)
+        self._size = lib().pw_size(self._h)
+
+    def setNu(self, nu: float):
+        self._nu = nu
+        self._ck(lib().pw_set_nu(self._h, nu))
+
+    # interaction.cpp:41-43
+    def solve(self):
+        self._ck(lib().pw_solve(self._h))
+
+    # interaction.cpp:204-229
+    def setStates(self, state):
+        s = _f64(state)
+        assert s.size == self._size
+        self._ck(lib().pw_set_state(self._h, _lib.ptr(s)))
+
+    def getStates(self, state=None):
+        out = np.empty(self._size) if state is None else state
+        self._ck(lib().pw_get_state(self._h, _lib.ptr(out)))
+        return out
+
+    def getRates(self, rate=None):
+        out = np.empty(self._size) if rate is None else rate
+        self._ck(lib().pw_get_rates(self._h, _lib.ptr(out)))
+        return out
+
+    def sync_wakes(self):
+        """Copy device positions, vorticities and last rates back into the
+        wakes' matrices."""
+        for k, W in enumerate(self._W):
+            self._ck(lib().pw_get_wake(self._h, k, _lib.ptr(W._position), _lib.ptr(W._vorticity),
+                                       _lib.ptr(W._velocity), _lib.ptr(W._retvorcity)))
+
+    def rk4(self, dt: float, nsteps: int = 1):
+        self._ck(lib().pw_rk4_steps(self._h, dt, nsteps))
+
+    def euler(self, dt: float, nsteps: int = 1):
+        self._ck(lib().pw_euler_steps(self._h, dt, nsteps))
+
+    def diagnose(self, ke: bool = True):
+        """Total vorticity, linear impulse (interaction.cpp:240-259) and
+        kinetic energy (interaction.cpp:351-383)."""
+        om = np.zeros(3)
+        im = np.zeros(3)
+        k = C.c_double(0.0)
+        self._ck(lib().pw_diagnostics(self._h, _lib.ptr(om), _lib.ptr(im),
+                                      C.cast(C.byref(k), C.POINTER(C.c_double)) if ke else None))
+        return {"total_vorticity": om, "linear_impulse": im,
+                "kinetic_energy": k.value if ke else None}
+
+    def last_timing(self):
+        t, p, n = C.c_double(), C.c_double(), C.c_double()
+        dp = C.POINTER(C.c_double)
+        lib().pw_last_timing(self._h, C.cast(C.byref(t), dp), C.cast(C.byref(p), dp),
+                             C.cast(C.byref(n), dp))
+        return {"total_ms": t.value, "pair_ms": p.value, "pairs": n.value}
+
+
+class Integration:
+    """pawan::__integration (integration.h:20-58): forward Euler stepping."""
+
+    def __init__(self, t: float, n: int):
+        self._t, self._n = t, n
+        self._dt = t / n  # integration.cpp:10-14
+
+    def step(self, dt: float, S: Interaction):
+        S.euler(dt, 1)
+
+    def integrate(self, S: Interaction, writer=None):
+        """Advance S by n steps; ``writer(t, S)`` is called at t = 0 and after
+        every step, as integrate() writes a frame per step
+        (integration.cpp:16-35)."""
+        t = 0.0
+        if writer:
+            writer(t, S)
+        for i in range(1, self._n + 1):
+            t = i * self._dt
+            self.step(self._dt, S)
+            if writer:
+                writer(t, S)
+
+
+class RK4(Integration):
+    """pawan::__rk4 (rk4.h:14-41): classic RK4, stages fused on device."""
+
+    def step(self, dt: float, S: Interaction):
+        S.rk4(dt, 1)
+
+    def integrate(self, S: Interaction, writer=None):
+        if writer is None:
+            S.rk4(self._dt, self._n)
+            return
+        super().integrate(S, writer)
+
+
+class Batch:
+    """Many independent simulations stepped in one persistent launch."""
+
+    def __init__(self, device: int = 0, kernel: int = _lib.KERNEL_HIGHORDER, block: int = 0):
+        h = C.c_void_p()
+        rc = lib().pw_batch_create(C.byref(h), device, kernel, block)
+        if rc != _lib.PW_OK:
+            raise _lib.PawanError(rc, lib().pw_batch_last_error(None).decode())
+        self._h = h
+        self.n = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pw_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        _check(rc, self._h, "pw_batch_last_error")
+
+    def add(self, w: dict, nu: float, dt: float, precision: int = _lib.FP64, uinf=(0, 0, 0)):
+        spec, keep = job_spec(w, nu, dt, precision, uinf)
+        jid = C.c_int64()
+        self._ck(lib().pw_batch_add(self._h, C.byref(spec), C.byref(jid)))
+        self.n[jid.value] = int(w["pos"].shape[0])
+        return jid.value
+
+    def remove(self, jid):
+        self._ck(lib().pw_batch_remove(self._h, jid))
+        self.n.pop(jid, None)
+
+    def step(self, nsteps: int = 1, graph: bool = True):
+        self._ck(lib().pw_batch_step(self._h, nsteps, 1 if graph else 0))
+
+    def solve(self):
+        self._ck(lib().pw_batch_solve(self._h))
+
+    def get(self, jid):
+        n = self.n[jid]
+        out = [np.empty((n, 3)) for _ in range(4)]
+        self._ck(lib().pw_batch_get(self._h, jid, *[_lib.ptr(a) for a in out]))
+        return {"pos": out[0], "vort": out[1], "vel": out[2], "dvort": out[3]}
+
+    def status(self, jid):
+        return lib().pw_batch_status(self._h, jid)
+
+    def last_timing(self):
+        t, p, n = C.c_double(), C.c_double(), C.c_double()
+        dp = C.POINTER(C.c_double)
+        lib().pw_batch_last_timing(self._h, C.cast(C.byref(t), dp), C.cast(C.byref(p), dp),
+                                   C.cast(C.byref(n), dp))
+        return {"total_ms": t.value, "pair_ms": p.value, "pairs": n.value}
+
+
+def job_spec(w: dict, nu: float, dt: float, precision: int, uinf=(0, 0, 0)):
+    """pw_job_spec for a generated particle set; returns (spec, arrays to keep alive)."""
+    keep = [_f64(w[k]) for k in ("pos", "vort", "radius", "volume")]
+    spec = _lib.PwJobSpec()
+    spec.n = keep[0].shape[0]
+    spec.pos, spec.vort, spec.radius, spec.volume = [_lib.ptr(a) for a in keep]
+    spec.nu, spec.dt, spec.precision = nu, dt, precision
+    spec.uinf[:] = list(uinf)
+    return spec, keep
+
+
+class Server:
+    """Continuous-batching request loop (pw_server_*)."""
+
+    QUEUED, RUNNING, DONE, FAILED, CANCELLED = 0, 1, 2, 3, 4
+
+    def __init__(self, hbm_budget_bytes: int, device: int = 0,
+                 kernel: int = _lib.KERNEL_HIGHORDER):
+        h = C.c_void_p()
+        rc = lib().pw_server_create(C.byref(h), device, kernel, hbm_budget_bytes)
+        if rc != _lib.PW_OK:
+            raise _lib.PawanError(rc, lib().pw_server_last_error(None).decode())
+        self._h = h
+        self.n = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pw_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        _check(rc, self._h, "pw_server_last_error")
+
+    def now_ms(self):
+        return lib().pw_server_now_ms(self._h)
+
+    def submit(self, w, nu, dt, nsteps, precision=_lib.FP64, uinf=(0, 0, 0), arrival_ms=-1.0):
+        spec, keep = job_spec(w, nu, dt, precision, uinf)
+        jid = C.c_int64()
+        self._ck(lib().pw_server_submit(self._h, C.byref(spec), nsteps, arrival_ms, C.byref(jid)))
+        self.n[jid.value] = int(w["pos"].shape[0])
+        return jid.value
+
+    def cancel(self, jid):
+        self._ck(lib().pw_server_cancel(self._h, jid))
+
+    def step(self):
+        r = C.c_int64()
+        self._ck(lib().pw_server_step(self._h, C.byref(r)))
+        return r.value
+
+    def state(self, jid):
+        d = C.c_int64()
+        s = lib().pw_server_job_state(self._h, jid, C.byref(d))
+        return s, d.value
+
+    def fetch(self, jid):
+        n = self.n[jid]
+        pos, vort = np.empty((n, 3)), np.empty((n, 3))
+        self._ck(lib().pw_server_fetch(self._h, jid, _lib.ptr(pos), _lib.ptr(vort)))
+        self.n.pop(jid)
+        return pos, vort
+
+    def stats(self):
+        st = _lib.PwServerStats()
+        self._ck(lib().pw_server_get_stats(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_}
