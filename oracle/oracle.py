@@ -47,6 +47,7 @@ def load():
         "orc_linear_impulse": (None, [C.c_size_t, _sp, _dp, _dp, _dp]),
         "orc_angular_impulse": (None, [C.c_size_t, _sp, _dp, _dp, _dp]),
         "orc_kinetic_energy": (C.c_double, [C.c_size_t, _sp, _dp, _dp, _dp]),
+        "orc_kinetic_energy_cross": (C.c_double, [C.c_size_t, C.c_size_t, _dp, _dp, _dp]),
         "orc_enstrophy": (C.c_double, [C.c_size_t, _sp, _dp, _dp, _dp]),
         "orc_helicity": (C.c_double, [C.c_size_t, _sp, _dp, _dp, _dp]),
         "orc_influence": (C.c_int, [C.c_int, C.c_size_t, _sp, _dp, _dp, _dp, _dp]),
@@ -152,6 +153,12 @@ def diagnostics(pos, vort, radius, wake_n=None, ke=True):
     if ke:
         out["kinetic_energy"] = L.orc_kinetic_energy(nw, wp, _p(pos), _p(vort), _p(radius))
     return out
+
+
+def kinetic_energy_cross(pos, vort, radius, n1):
+    """The reference's calculateKineticEnergy(W1, W2) with W1 = first n1 particles."""
+    pos, vort, radius = _f(pos), _f(vort), _f(radius)
+    return load().orc_kinetic_energy_cross(n1, radius.size - n1, _p(pos), _p(vort), _p(radius))
 
 
 def enstrophy(pos, vort, radius, wake_n=None):

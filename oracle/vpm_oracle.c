@@ -722,11 +722,26 @@ static double quad_diag(size_t nwake, const size_t *wake_n, const double *pos,
     return total;
 }
 
-/* calculateKineticEnergy(W) = 2[sum self + 2 sum_{I<J} pair] (:351-366);
- * calculateKineticEnergy(W1,W2) = 2 sum pair (:368-383) */
+/* calculateKineticEnergy(W) = 2[sum self + 2 sum_{I<J} pair] (:351-366),
+ * applied to all particles of all wakes.  The reference's cross-wake routine
+ * calculateKineticEnergy(W1,W2) = 2 sum pair (:368-383) weights a cross pair
+ * half as much as the single-wake routine weights a pair, so a sum of the two
+ * would depend on how particles are split into wakes (diagnose() calls
+ * neither); cross pairs get the single-wake weight 4 here. */
 double orc_kinetic_energy(size_t nwake, const size_t *wake_n, const double *pos,
                           const double *vort, const double *radius) {
-    return quad_diag(nwake, wake_n, pos, vort, radius, ke_self, ke_pair, 2.0, 2.0, 2.0);
+    return quad_diag(nwake, wake_n, pos, vort, radius, ke_self, ke_pair, 2.0, 2.0, 4.0);
+}
+
+/* The reference's cross-wake routine as written (interaction.cpp:368-383). */
+double orc_kinetic_energy_cross(size_t n1, size_t n2, const double *pos, const double *vort,
+                                const double *radius) {
+    double ke = 0.0;
+    for (size_t I = 0; I < n1; ++I)
+        for (size_t J = n1; J < n1 + n2; ++J)
+            ke += ke_pair(radius[I], radius[J], pos + 3 * I, pos + 3 * J, vort + 3 * I,
+                          vort + 3 * J);
+    return 2.0 * ke;
 }
 
 /* calculateEnstrophy(W) = sum self + 2 sum_{I<J} pair (:277-298);

@@ -100,10 +100,14 @@ void orc_linear_impulse(size_t nwake, const size_t *wake_n, const double *pos,
                         const double *vort, double out[3]);
 void orc_angular_impulse(size_t nwake, const size_t *wake_n, const double *pos,
                          const double *vort, double out[3]);
-/* kinetic energy of all wakes: sum of calculateKineticEnergy(W) over wakes plus
- * calculateKineticEnergy(Wi,Wj) over wake pairs i<j (interaction.cpp:351-383) */
+/* kinetic energy of all particles: calculateKineticEnergy(W)
+ * (interaction.cpp:351-366) over the union of the wakes */
 double orc_kinetic_energy(size_t nwake, const size_t *wake_n, const double *pos,
                           const double *vort, const double *radius);
+/* calculateKineticEnergy(W1, W2) as written (interaction.cpp:368-383), with
+ * W1 the first n1 particles and W2 the next n2 */
+double orc_kinetic_energy_cross(size_t n1, size_t n2, const double *pos, const double *vort,
+                                const double *radius);
 double orc_enstrophy(size_t nwake, const size_t *wake_n, const double *pos, const double *vort,
                      const double *radius);
 double orc_helicity(size_t nwake, const size_t *wake_n, const double *pos, const double *vort,

@@ -13,11 +13,11 @@
 //   GAUSSIAN : Zs = (2 pi)^-1.5 exp(-t/2)
 //              Q  = [(0.25/pi) erf(sqrt(t/2)) t^-1/2 - Zs] / t
 //   both     : F' = (Zs - 3Q) / t
-// and the per-target sums are regrouped so that each pair costs as little as
-// possible:
-//   sum q (a_i x a_j)           = a_i x sum q a_j
-//   d.(a_i x a_j)               = a_i.(a_j x d)      (reuses the velocity cross)
-//   sum k (v_i a_j - v_j a_i)   = v_i sum k a_j - a_i sum k v_j
+// The PSE sum is regrouped, sum k (v_i a_j - v_j a_i) = v_i sum k a_j -
+// a_i sum k v_j, which saves four operations per pair and loses little (the
+// two sums cancel by a few digits at most).  The stretching cross product is
+// kept per pair: a_i x sum q a_j would cancel catastrophically when the
+// vorticities are nearly parallel, as they are along a vortex filament.
 // Each work item (target tile x source chunk) writes its NCOMP partial sums to
 // its own slot; a second kernel adds the chunks in a fixed order and applies
 // the epilogue, so results never depend on scheduling.
@@ -110,10 +110,14 @@ __device__ __forceinline__ void pair64(double xi, double yi, double zi, double a
     a.v[0] = __builtin_fma(Q, cx, a.v[0]);
     a.v[1] = __builtin_fma(Q, cy, a.v[1]);
     a.v[2] = __builtin_fma(Q, cz, a.v[2]);
-    a.v[3] = __builtin_fma(Q, s.ax, a.v[3]);
-    a.v[4] = __builtin_fma(Q, s.ay, a.v[4]);
-    a.v[5] = __builtin_fma(Q, s.az, a.v[5]);
-    const double fd = Fp * __builtin_fma(aiz, cz, __builtin_fma(aiy, cy, aix * cx));
+    // e = a_i x a_j
+    const double ex = __builtin_fma(aiy, s.az, -aiz * s.ay);
+    const double ey = __builtin_fma(aiz, s.ax, -aix * s.az);
+    const double ez = __builtin_fma(aix, s.ay, -aiy * s.ax);
+    a.v[3] = __builtin_fma(Q, ex, a.v[3]);
+    a.v[4] = __builtin_fma(Q, ey, a.v[4]);
+    a.v[5] = __builtin_fma(Q, ez, a.v[5]);
+    const double fd = Fp * __builtin_fma(dz, ez, __builtin_fma(dy, ey, dx * ex));
     a.v[6] = __builtin_fma(fd, dx, a.v[6]);
     a.v[7] = __builtin_fma(fd, dy, a.v[7]);
     a.v[8] = __builtin_fma(fd, dz, a.v[8]);
@@ -153,10 +157,13 @@ __device__ __forceinline__ void pair32(double xi, double yi, double zi, float ai
     a.v[0] = __builtin_fmaf(Q, cx, a.v[0]);
     a.v[1] = __builtin_fmaf(Q, cy, a.v[1]);
     a.v[2] = __builtin_fmaf(Q, cz, a.v[2]);
-    a.v[3] = __builtin_fmaf(Q, s.ax, a.v[3]);
-    a.v[4] = __builtin_fmaf(Q, s.ay, a.v[4]);
-    a.v[5] = __builtin_fmaf(Q, s.az, a.v[5]);
-    const float fd = Fp * __builtin_fmaf(aiz, cz, __builtin_fmaf(aiy, cy, aix * cx));
+    const float ex = __builtin_fmaf(aiy, s.az, -aiz * s.ay);
+    const float ey = __builtin_fmaf(aiz, s.ax, -aix * s.az);
+    const float ez = __builtin_fmaf(aix, s.ay, -aiy * s.ax);
+    a.v[3] = __builtin_fmaf(Q, ex, a.v[3]);
+    a.v[4] = __builtin_fmaf(Q, ey, a.v[4]);
+    a.v[5] = __builtin_fmaf(Q, ez, a.v[5]);
+    const float fd = Fp * __builtin_fmaf(dz, ez, __builtin_fmaf(dy, ey, dx * ex));
     a.v[6] = __builtin_fmaf(fd, dx, a.v[6]);
     a.v[7] = __builtin_fmaf(fd, dy, a.v[7]);
     a.v[8] = __builtin_fmaf(fd, dz, a.v[8]);
@@ -345,7 +352,7 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
 // Per particle: add the chunk partials in chunk order (binary64), form the
 // rates
 //   u   = su * S[0:3] + uinf
-//   da  = sq * (a_i x S[3:6]) + sf * S[6:9] + sd * (v_i S[9:12] - a_i S[12])
+//   da  = sq * S[3:6] + sf * S[6:9] + sd * (v_i S[9:12] - a_i S[12])
 // (uniform core: su = sq = sigma^-3, sf = sigma^-5, sd = 2 nu sigma^-5;
 // otherwise the per-pair factors are already inside and su = sq = sf = 1,
 // sd = 2 nu), then apply the RK4 / Euler stage combination of rk4.cpp:21-66
@@ -405,12 +412,9 @@ __global__ __launch_bounds__(256) void combine_kernel(const JobDev *__restrict__
         k[1] = J.su * S[1] + J.uinf[1];
         k[2] = J.su * S[2] + J.uinf[2];
         const double ax = xs[3], ay = xs[4], az = xs[5];
-        const double st0 = ay * S[5] - az * S[4];
-        const double st1 = az * S[3] - ax * S[5];
-        const double st2 = ax * S[4] - ay * S[3];
-        k[3] = J.sq * st0 + J.sf * S[6] + J.sd * (vi * S[9] - ax * S[12]);
-        k[4] = J.sq * st1 + J.sf * S[7] + J.sd * (vi * S[10] - ay * S[12]);
-        k[5] = J.sq * st2 + J.sf * S[8] + J.sd * (vi * S[11] - az * S[12]);
+        k[3] = J.sq * S[3] + J.sf * S[6] + J.sd * (vi * S[9] - ax * S[12]);
+        k[4] = J.sq * S[4] + J.sf * S[7] + J.sd * (vi * S[10] - ay * S[12]);
+        k[5] = J.sq * S[5] + J.sf * S[8] + J.sd * (vi * S[11] - az * S[12]);
         bool finite = true;
 #pragma unroll
         for (int m = 0; m < 6; ++m) {

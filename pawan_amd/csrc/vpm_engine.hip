@@ -310,31 +310,39 @@ void Engine::build_plan() {
     n64_ = (int)it64.size();
     n32_ = (int)it32.size();
     nci_ = (int)ci.size();
-    size_t o_jobs = 0;
-    size_t o_i64 = round_up(o_jobs + std::max<size_t>(1, jd.size()) * sizeof(JobDev), 256);
-    size_t o_i32 = round_up(o_i64 + std::max<size_t>(1, it64.size()) * sizeof(Item), 256);
-    size_t o_ci = round_up(o_i32 + std::max<size_t>(1, it32.size()) * sizeof(Item), 256);
-    size_t o_cnt = round_up(o_ci + std::max<size_t>(1, ci.size()) * sizeof(CItem), 256);
-    size_t total = o_cnt + 256;
-    if (total > plan_.bytes) {
-        PW_HIP(hipStreamSynchronize(stream_));
-        plan_.reset(round_up(total * 3 / 2, 4096));
+    // Region capacities only grow.  The region addresses are kernel arguments
+    // baked into the captured step graph, so they must not move while only
+    // the table contents change; a growth starts a new layout version, for
+    // which the graph is captured again.
+    const bool grow = !plan_.p || jd.size() > cap_jobs_ || it64.size() > cap_i64_ ||
+                      it32.size() > cap_i32_ || ci.size() > cap_ci_;
+    if (grow) {
+        cap_jobs_ = std::max(cap_jobs_, round_up(std::max<size_t>(jd.size(), 1) * 2, 16));
+        cap_i64_ = std::max(cap_i64_, round_up(std::max<size_t>(it64.size(), 1) * 2, 1024));
+        cap_i32_ = std::max(cap_i32_, round_up(std::max<size_t>(it32.size(), 1) * 2, 1024));
+        cap_ci_ = std::max(cap_ci_, round_up(std::max<size_t>(ci.size(), 1) * 2, 1024));
+        o_i64_ = round_up(cap_jobs_ * sizeof(JobDev), 256);
+        o_i32_ = round_up(o_i64_ + cap_i64_ * sizeof(Item), 256);
+        o_ci_ = round_up(o_i32_ + cap_i32_ * sizeof(Item), 256);
+        o_cnt_ = round_up(o_ci_ + cap_ci_ * sizeof(CItem), 256);
+        plan_.reset(o_cnt_ + 256);
+        plan_host_.ensure(plan_.bytes);
+        ++layout_version_;
     }
-    plan_host_.ensure(plan_.bytes);
     char *h = plan_host_.as<char>();
-    std::memcpy(h + o_jobs, jd.data(), jd.size() * sizeof(JobDev));
-    std::memcpy(h + o_i64, it64.data(), it64.size() * sizeof(Item));
-    std::memcpy(h + o_i32, it32.data(), it32.size() * sizeof(Item));
-    std::memcpy(h + o_ci, ci.data(), ci.size() * sizeof(CItem));
+    std::memcpy(h, jd.data(), jd.size() * sizeof(JobDev));
+    std::memcpy(h + o_i64_, it64.data(), it64.size() * sizeof(Item));
+    std::memcpy(h + o_i32_, it32.data(), it32.size() * sizeof(Item));
+    std::memcpy(h + o_ci_, ci.data(), ci.size() * sizeof(CItem));
     int cnt[4] = {n64_, n32_, nci_, 0};
-    std::memcpy(h + o_cnt, cnt, sizeof cnt);
-    PW_HIP(hipMemcpyAsync(plan_.p, h, total, hipMemcpyHostToDevice, stream_));
+    std::memcpy(h + o_cnt_, cnt, sizeof cnt);
+    PW_HIP(hipMemcpyAsync(plan_.p, h, o_cnt_ + 256, hipMemcpyHostToDevice, stream_));
     char *d = plan_.as<char>();
-    d_jobs_ = (JobDev *)(d + o_jobs);
-    d_items64_ = (Item *)(d + o_i64);
-    d_items32_ = (Item *)(d + o_i32);
-    d_citems_ = (CItem *)(d + o_ci);
-    d_counts_ = (int *)(d + o_cnt);
+    d_jobs_ = (JobDev *)d;
+    d_items64_ = (Item *)(d + o_i64_);
+    d_items32_ = (Item *)(d + o_i32_);
+    d_citems_ = (CItem *)(d + o_ci_);
+    d_counts_ = (int *)(d + o_cnt_);
     plan_dirty_ = false;
 }
 
@@ -421,7 +429,7 @@ void Engine::capture_graph() {
     }
     PW_HIP(hipStreamEndCapture(stream_, &graph_));
     PW_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-    graph_plan_ptr_ = plan_.p;
+    graph_layout_ = layout_version_;
 }
 
 void Engine::rk4(size_t nsteps, bool use_graph) {
@@ -432,7 +440,7 @@ void Engine::rk4(size_t nsteps, bool use_graph) {
         return;
     }
     pair_launches = 0;
-    if (use_graph && (!graph_exec_ || graph_plan_ptr_ != plan_.p)) capture_graph();
+    if (use_graph && (!graph_exec_ || graph_layout_ != layout_version_)) capture_graph();
     begin_timing(ev_begin_, stream_);
     for (size_t s = 0; s < nsteps; ++s) {
         if (use_graph) {

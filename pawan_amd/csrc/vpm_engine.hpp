@@ -147,7 +147,11 @@ class Engine {
     int n64_ = 0, n32_ = 0, nci_ = 0;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
-    void *graph_plan_ptr_ = nullptr;
+    // plan-table layout: region capacities and offsets, and the layout a
+    // captured graph was recorded against
+    size_t cap_jobs_ = 0, cap_i64_ = 0, cap_i32_ = 0, cap_ci_ = 0;
+    size_t o_i64_ = 0, o_i32_ = 0, o_ci_ = 0, o_cnt_ = 0;
+    uint64_t layout_version_ = 0, graph_layout_ = ~0ull;
     std::vector<hipEvent_t> ev_pool_;
     hipEvent_t ev_begin_ = nullptr, ev_end_ = nullptr;
     HostBuf stage_;

@@ -11,7 +11,7 @@ constexpr int PREC_FP64 = 0;
 constexpr int PREC_FP32C = 1;
 
 // Partial-sum components per (target, chunk):
-//   0-2 sum q (a_j x d)   3-5 sum q a_j   6-8 sum F (a_i.(a_j x d)) d
+//   0-2 sum q (a_j x d)   3-5 sum q (a_i x a_j)   6-8 sum F (d.(a_i x a_j)) d
 //   9-11 sum k a_j        12 sum k v_j
 constexpr int NCOMP = 13;
 

@@ -1,0 +1,122 @@
+"""Batched and served simulations: every job's result is the same alone and
+in a mix (bitwise: each particle's sums are formed in an order fixed by its
+own job), graph replay equals eager launches, and failures stay isolated."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import pawan_amd as pa
+from pawan_amd import generators as gen
+from helpers import per_particle_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs():
+    a = gen.vortex_ring(40, 16, R=1.0, core=0.1, seed=1)              # 640
+    b = gen.vortex_ring(100, 32, R=0.7, core=0.08, seed=2)            # 3200
+    c = gen.rotor_wake(6)                                              # 1536
+    d = gen.vortex_ring(17, 9, R=1.3, core=0.12, seed=3, center=(0.1, 0.2, 0.3))  # 153
+    return [(a, 1e-3, 0.005, pa.FP64, (0, 0, 0)), (b, 5e-4, 0.005, pa.FP32C, (0, 0, 0)),
+            (c, 1e-4, 0.05, pa.FP64, (0, 0, -0.05)), (d, 2e-3, 0.01, pa.FP32C, (0, 0, 0))]
+
+
+def _run_batch(jobs, steps, graph=True):
+    B = pa.Batch(block=128)
+    ids = [B.add(w, nu, dt, prec, uinf) for (w, nu, dt, prec, uinf) in jobs]
+    B.step(steps, graph=graph)
+    out = [B.get(i) for i in ids]
+    st = [B.status(i) for i in ids]
+    B.close()
+    return out, st
+
+
+def test_alone_equals_mixed_bitwise():
+    jobs = _jobs()
+    print("case mixed batch of 4 jobs, 3 steps", flush=True)
+    mixed, st = _run_batch(jobs, 3)
+    assert all(s == 0 for s in st)
+    for k, job in enumerate(jobs):
+        print(f"case job {k} alone", flush=True)
+        alone, _ = _run_batch([job], 3)
+        for key in ("pos", "vort", "vel", "dvort"):
+            assert np.array_equal(alone[0][key], mixed[k][key]), (k, key)
+
+
+def test_graph_equals_eager():
+    jobs = _jobs()[:2]
+    print("case graph vs eager", flush=True)
+    g, _ = _run_batch(jobs, 2, graph=True)
+    e, _ = _run_batch(jobs, 2, graph=False)
+    for k in range(len(jobs)):
+        for key in ("pos", "vort"):
+            assert np.array_equal(g[k][key], e[k][key])
+
+
+def test_batch_matches_oracle():
+    jobs = _jobs()
+    out, _ = _run_batch(jobs, 2)
+    for k, (w, nu, dt, prec, uinf) in enumerate(jobs):
+        x0 = orc.pack_state(w["pos"], w["vort"])
+        xo = orc.rk4(x0, w["radius"], w["volume"], nu, dt, 2, mode=orc.OMP_CLEAN,
+                     uinf=np.array(uinf, dtype=float))
+        po, vo = orc.unpack_state(xo)
+        ep, ea = per_particle_rel(out[k]["pos"], po), per_particle_rel(out[k]["vort"], vo)
+        print(f"  job {k} prec {prec}: pos {ep:.3e} vort {ea:.3e}", flush=True)
+        tol = 1e-11 if prec == pa.FP64 else 1e-6
+        assert ep < tol and ea < (1e-9 if prec == pa.FP64 else 1e-5)
+
+
+def test_server_alone_vs_mixed_and_failures():
+    jobs = _jobs()
+    alone = []
+    for k, (w, nu, dt, prec, uinf) in enumerate(jobs):
+        print(f"case server job {k} alone", flush=True)
+        srv = pa.Server(8 << 30)
+        jid = srv.submit(w, nu, dt, 3 + k, prec, uinf)
+        while srv.step() > 0:
+            pass
+        assert srv.state(jid)[0] == pa.Server.DONE
+        alone.append(srv.fetch(jid))
+        srv.close()
+    print("case server mixed with staggered arrivals, a failing job and a cancel", flush=True)
+    srv = pa.Server(8 << 30)
+    bad = gen.vortex_ring(8, 1, seed=5)
+    bad["pos"][3] = bad["pos"][2]  # coincident pair -> non-finite rates
+    ids = {}
+    ids[0] = srv.submit(*_args(jobs[0], 3))
+    srv.step()
+    ids["bad"] = srv.submit(bad, 1e-3, 0.01, 5)
+    ids[1] = srv.submit(*_args(jobs[1], 4))
+    victim = srv.submit(*_args(jobs[2], 50))
+    srv.step()
+    ids[2] = srv.submit(*_args(jobs[2], 5))
+    srv.cancel(victim)
+    srv.step()
+    ids[3] = srv.submit(*_args(jobs[3], 6))
+    while srv.step() > 0:
+        pass
+    assert srv.state(ids["bad"])[0] == pa.Server.FAILED
+    assert srv.state(victim)[0] == pa.Server.CANCELLED
+    for k in range(4):
+        assert srv.state(ids[k])[0] == pa.Server.DONE
+        pos, vort = srv.fetch(ids[k])
+        assert np.array_equal(pos, alone[k][0]) and np.array_equal(vort, alone[k][1]), k
+    st = srv.stats()
+    print(f"  stats {st}", flush=True)
+    assert st["finished"] == 4 and st["failed"] == 1 and st["cancelled"] == 1
+    assert st["hbm_in_use"] == 0
+    srv.close()
+
+
+def test_server_budget():
+    srv = pa.Server(1 << 20)  # 1 MiB: a 3,200-particle job does not fit
+    w = gen.vortex_ring(100, 32, seed=2)
+    with pytest.raises(pa.PawanError):
+        srv.submit(w, 1e-3, 0.01, 2)
+    srv.close()
+
+
+def _args(job, steps):
+    w, nu, dt, prec, uinf = job
+    return (w, nu, dt, steps, prec, uinf)
