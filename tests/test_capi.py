@@ -1,0 +1,83 @@
+"""The C ABI library: loads, exports every function include/pawan_hip.h
+declares, and its host-only entry points behave (no GPU needed)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import pawan_amd
+from pawan_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pawan_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pw_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_api():
+    names = declared_functions()
+    assert len(names) >= 35
+    for must in ("pw_create", "pw_solve", "pw_rk4_steps", "pw_batch_step", "pw_server_step"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_declared_symbol():
+    assert set(declared_functions()) == set(_lib.SIGNATURES)
+
+
+def test_host_only_entry_points():
+    lib = _lib.load()
+    assert lib.pw_version().decode().startswith("pawan_hip")
+    cfg = _lib.PwConfig()
+    assert lib.pw_config_default(C.byref(cfg)) == 0
+    assert cfg.nu == 2.5e-3 and cfg.kernel == 0 and cfg.precision == 0
+    # chunk policy: multiple of the block, at most 16 chunks, function of (n, block) only
+    for n in (1, 100, 1024, 2048, 65536, 262144, 1000003):
+        for b in (64, 128, 256):
+            L = lib.pw_chunk_for(n, b)
+            assert L % b == 0 and (n + L - 1) // L <= 16
+    assert lib.pw_job_footprint(262144, 0) > 262144 * 64
+    assert lib.pw_job_footprint(262144, 1) < lib.pw_job_footprint(262144, 0)
+
+
+def test_null_arguments_rejected():
+    lib = _lib.load()
+    assert lib.pw_create(None, None) == _lib.PW_EINVAL
+    assert lib.pw_destroy(None) == _lib.PW_EINVAL
+    assert lib.pw_solve(None) == _lib.PW_EINVAL
+    assert lib.pw_rk4_steps(None, 0.01, 1) == _lib.PW_EINVAL
+    assert lib.pw_batch_step(None, 1, 1) == _lib.PW_EINVAL
+    assert lib.pw_server_step(None, None) == _lib.PW_EINVAL
+    assert lib.pw_config_default(None) == _lib.PW_EINVAL
+
+
+def test_invalid_config_rejected_before_device_use():
+    lib = _lib.load()
+    cfg = _lib.PwConfig()
+    lib.pw_config_default(C.byref(cfg))
+    h = C.c_void_p()
+    for field, bad in (("kernel", 7), ("precision", 5), ("block", 96)):
+        c2 = _lib.PwConfig.from_buffer_copy(cfg)
+        setattr(c2, field, bad)
+        assert lib.pw_create(C.byref(h), C.byref(c2)) == _lib.PW_EINVAL
+        assert h.value is None
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_no_device_is_an_error_not_a_crash():
+    lib = _lib.load()
+    h = C.c_void_p()
+    rc = lib.pw_create(C.byref(h), None)
+    assert rc in (_lib.PW_EHIP, _lib.PW_EINVAL) and h.value is None
+    assert lib.pw_last_error(None)
