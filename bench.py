@@ -35,6 +35,10 @@ sys.path.insert(0, ROOT)
 FLOP_PER_PAIR = 95.0          # algorithmic FLOP per ordered pair (one-sided form)
 FP64_VECTOR_PEAK = 78.6e12    # MI355X FP64 vector FLOP/s
 FP32_VECTOR_PEAK = 157.3e12   # MI355X FP32 vector FLOP/s (published, packed)
+# HBM bytes per binary64 pair-kernel launch at N = 262,144, from rocprofv3
+# FETCH_SIZE + WRITE_SIZE passes (KiB per dispatch, write factor 1.00 from a
+# copy kernel of known size); profiles/pair_kernels_rotor_wake_262144.md
+PAIR64_TRAFFIC_262144 = (229_000 + 426_528) * 1024.0
 
 
 def log(*a):
@@ -206,7 +210,8 @@ def main():
                    "shed_steps": args.n_shed, "core": 0.02, "nu": 1e-4, "uinf_z": -0.05,
                    "step": "one RK4 step = 4 all-pairs RHS", "dt": STEP_DT, "pairs_per_step": 4.0 * n * (n - 1)},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": FP64_VECTOR_PEAK / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_VECTOR_PEAK, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_VECTOR_PEAK,
+                     "traffic": PAIR64_TRAFFIC_262144 if n == 262144 else None,
                      "kernel": "pair64_kernel", "flop_per_pair": FLOP_PER_PAIR,
                      "launch_ms": r64["pair_kernel_ms_mean"]},
     }
