@@ -1,0 +1,181 @@
+// pawan_gpu.hpp -- drop-in C++ adapter: pawan's interaction and RK4 classes
+// backed by the gfx950 C ABI (pawan_hip.h).
+//
+// The adapter is a template over pawan's own classes so that the same header
+// compiles inside pawan's tree:
+//
+//   using __gpu_interaction = pawan_gpu::interaction<pawan::__interaction, pawan::__wake, gsl_vector>;
+//   using __gpu_rk4 = pawan_gpu::rk4<pawan::__rk4, pawan::__system, gsl_vector>;
+//
+// Requirements on the template arguments (all met by pawan's classes):
+//   Interaction  virtual addWake(Wake*), solve(), write(FILE*), diagnose(),
+//                setStates(const Vector*), getRates(Vector*), getStates(Vector*)
+//                (interaction.h:181-234); protected double _nu (interaction.h:24)
+//   Wake         _numParticles; _position, _vorticity, _velocity, _retvorcity
+//                with ->data holding N x 3 row-major doubles (tda == 3);
+//                _radius, _volume with ->data holding N doubles (wake.h:33-42)
+//   Vector       ->data holding the state/rate vector (gsl_vector, stride 1)
+//   RK4          RK4(const double &t, const size_t &n); protected virtual
+//                step(const double &dt, System *S, Vector *state) (rk4.h:14-41)
+//
+// Semantics follow the reference: solve() fills every wake's _velocity and
+// _retvorcity; setStates() also updates the wakes' _position/_vorticity;
+// write() and diagnose() see the device state.  RK4 steps run entirely on
+// the device (pw_rk4_steps); the integrator's state vector is refreshed after
+// each step, as __rk4::step leaves it.
+#pragma once
+
+#include <cstddef>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pawan_hip.h"
+
+namespace pawan_gpu {
+
+inline void check(int rc, const pw_ctx *ctx) {
+    if (rc != PW_OK)
+        throw std::runtime_error(std::string("pawan_hip: ") + pw_last_error(ctx) + " (status " +
+                                 std::to_string(rc) + ")");
+}
+
+// Common base of device-backed systems, so an integrator can reach the context
+// through the reference's __system pointer.
+class device_system {
+  public:
+    virtual ~device_system() = default;
+    virtual pw_ctx *context() = 0;
+    // copy device positions, vorticities and last rates into the wakes
+    virtual void pull() = 0;
+};
+
+struct options {
+    double nu = 2.5e-3;  // reference default, interaction.cpp:13
+    int precision = PW_FP64;
+    int kernel = PW_KERNEL_HIGHORDER;
+    int device = 0;
+    double uinf[3] = {0.0, 0.0, 0.0};
+};
+
+template <class Interaction, class Wake, class Vector>
+class interaction : public Interaction, public device_system {
+  public:
+    explicit interaction(const options &o = options()) : Interaction() {
+        pw_config c;
+        pw_config_default(&c);
+        c.nu = o.nu;
+        c.precision = o.precision;
+        c.kernel = o.kernel;
+        c.device = o.device;
+        for (int k = 0; k < 3; ++k) c.uinf[k] = o.uinf[k];
+        check(pw_create(&ctx_, &c), nullptr);
+        this->_nu = o.nu;
+    }
+    // __interaction(W) and __interaction(W1, W2), interaction.cpp:20-27
+    explicit interaction(Wake *W, const options &o = options()) : interaction(o) { addWake(W); }
+    interaction(Wake *W1, Wake *W2, const options &o = options()) : interaction(o) {
+        addWake(W1);
+        addWake(W2);
+    }
+    ~interaction() override { pw_destroy(ctx_); }
+    interaction(const interaction &) = delete;
+    interaction &operator=(const interaction &) = delete;
+
+    pw_ctx *context() override { return ctx_; }
+
+    // interaction.cpp:35-39, plus a copy of the particles to the device
+    void addWake(Wake *W) override {
+        Interaction::addWake(W);
+        wakes_.push_back(W);
+        check(pw_add_wake(ctx_, W->_numParticles, W->_position->data, W->_vorticity->data,
+                          W->_radius->data, W->_volume->data),
+              ctx_);
+    }
+
+    // interaction.cpp:41-43 -> interact(): rates of every particle
+    void solve() override {
+        check(pw_solve(ctx_), ctx_);
+        pull_rates();
+    }
+
+    // interaction.cpp:204-211
+    void setStates(const Vector *state) override {
+        Interaction::setStates(state);
+        check(pw_set_state(ctx_, state->data), ctx_);
+    }
+    // interaction.cpp:213-229
+    void getRates(Vector *rate) override { check(pw_get_rates(ctx_, rate->data), ctx_); }
+    void getStates(Vector *state) override { check(pw_get_state(ctx_, state->data), ctx_); }
+
+    // interaction.cpp:197-202: frames carry the device state
+    void write(FILE *f) override {
+        pull();
+        Interaction::write(f);
+    }
+    // interaction.cpp:49-93 on the device state
+    void diagnose() override {
+        pull();
+        Interaction::diagnose();
+    }
+
+    void pull() override {
+        for (size_t k = 0; k < wakes_.size(); ++k) {
+            Wake *W = wakes_[k];
+            check(pw_get_wake(ctx_, k, W->_position->data, W->_vorticity->data,
+                              W->_velocity->data, W->_retvorcity->data),
+                  ctx_);
+        }
+    }
+
+    // kinetic energy (interaction.cpp:351-383) with total vorticity and
+    // linear impulse, computed on the device
+    void device_diagnostics(double omega[3], double impulse[3], double *ke) {
+        check(pw_diagnostics(ctx_, omega, impulse, ke), ctx_);
+    }
+
+  private:
+    void pull_rates() {
+        for (size_t k = 0; k < wakes_.size(); ++k) {
+            Wake *W = wakes_[k];
+            check(pw_get_wake(ctx_, k, nullptr, nullptr, W->_velocity->data,
+                              W->_retvorcity->data),
+                  ctx_);
+        }
+    }
+    pw_ctx *ctx_ = nullptr;
+    std::vector<Wake *> wakes_;
+};
+
+// __rk4 whose step runs on the device (rk4.cpp:12-77 fused into pw_rk4_steps)
+template <class RK4, class System, class Vector>
+class rk4 : public RK4 {
+  public:
+    rk4(const double &t, const size_t &n) : RK4(t, n) {}
+
+  protected:
+    void step(const double &dt, System *S, Vector *states) override {
+        device_system *d = dynamic_cast<device_system *>(S);
+        if (!d) throw std::invalid_argument("pawan_gpu::rk4 steps a pawan_gpu::interaction");
+        check(pw_rk4_steps(d->context(), dt, 1), d->context());
+        check(pw_get_state(d->context(), states->data), d->context());
+    }
+};
+
+// forward Euler, integration.cpp:37-45
+template <class Integration, class System, class Vector>
+class euler : public Integration {
+  public:
+    euler(const double &t, const size_t &n) : Integration(t, n) {}
+
+  protected:
+    void step(const double &dt, System *S, Vector *states) override {
+        device_system *d = dynamic_cast<device_system *>(S);
+        if (!d) throw std::invalid_argument("pawan_gpu::euler steps a pawan_gpu::interaction");
+        check(pw_euler_steps(d->context(), dt, 1), d->context());
+        check(pw_get_state(d->context(), states->data), d->context());
+    }
+};
+
+}  // namespace pawan_gpu

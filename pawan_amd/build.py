@@ -23,6 +23,13 @@ def build_oracle() -> str:
     return os.path.join(ROOT, "oracle", "liboracle.so")
 
 
+def build_adapter_driver() -> str:
+    """The C++ drop-in adapter's test driver (needs both libraries above)."""
+    _make(os.path.join(ROOT, "tests", "cxx"))
+    return os.path.join(ROOT, "tests", "cxx", "adapter_driver")
+
+
 if __name__ == "__main__":
     build_library()
     build_oracle()
+    build_adapter_driver()

@@ -1,0 +1,124 @@
+// adapter_driver.cpp -- runs pawan's call sequence through the drop-in adapter:
+//   W = ring; S = new __gpu_interaction(W); S->solve(); read W->_velocity ...
+//   IN = new __gpu_rk4(t, n); IN->integrate(S, file)
+// and checks every result against the CPU oracle (liboracle.so).
+// Usage: adapter_driver <wake-file>   (exit status 0 = all checks passed)
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "../../include/pawan_gpu.hpp"
+#include "../../oracle/vpm_oracle.h"
+#include "mini_pawan.hpp"
+
+using gpu_interaction = pawan_gpu::interaction<mini::interaction, mini::wake, mini::vec>;
+using gpu_rk4 = pawan_gpu::rk4<mini::rk4, mini::system, mini::vec>;
+
+static mini::wake *ring(size_t n_azi, size_t n_cs, double R, double z0) {
+    auto *W = new mini::wake(n_azi * n_cs);
+    const double core = 0.1, h = 2 * core / 4.0, dpsi = 2 * M_PI / n_azi;
+    size_t p = 0;
+    for (size_t a = 0; a < n_azi; ++a)
+        for (size_t c = 0; c < n_cs; ++c, ++p) {
+            double xi = -core + h * (c % 4 + 0.5), eta = -core + h * (c / 4 + 0.5);
+            double psi = a * dpsi + 1e-3 * std::sin(7.0 * p), rho = R + xi;
+            W->_position->data[3 * p] = rho * std::cos(psi);
+            W->_position->data[3 * p + 1] = rho * std::sin(psi);
+            W->_position->data[3 * p + 2] = z0 + eta;
+            double g = std::exp(-(xi * xi + eta * eta) / (core * core)) / n_cs * rho * dpsi;
+            W->_vorticity->data[3 * p] = -g * std::sin(psi);
+            W->_vorticity->data[3 * p + 1] = g * std::cos(psi);
+            W->_vorticity->data[3 * p + 2] = 0.0;
+            W->_radius->data[p] = 1.3 * h;
+            W->_volume->data[p] = h * h * rho * dpsi;
+        }
+    return W;
+}
+
+static double relerr(const std::vector<double> &a, const std::vector<double> &b) {
+    double num = 0, den = 0;
+    for (size_t i = 0; i < a.size(); ++i) {
+        num = std::fmax(num, std::fabs(a[i] - b[i]));
+        den = std::fmax(den, std::fabs(b[i]));
+    }
+    return num / den;
+}
+
+int main(int argc, char **argv) {
+    const char *path = argc > 1 ? argv[1] : "adapter.wake";
+    mini::wake *W1 = ring(64, 16, 1.0, 0.0), *W2 = ring(48, 16, 0.8, 0.3);
+    const size_t n1 = W1->_numParticles, n2 = W2->_numParticles, n = n1 + n2;
+    pawan_gpu::options o;
+    o.nu = 1e-3;
+    int bad = 0;
+    try {
+        gpu_interaction S(W1, W2, o);  // __interaction(W1, W2)
+        std::printf("driver: N = %zu + %zu, _size = %zu\n", n1, n2, S._size);
+        // the oracle's view of the same particles
+        std::vector<double> pos(3 * n), vort(3 * n), rad(n), vol(n);
+        size_t wn[2] = {n1, n2};
+        for (int k = 0; k < 2; ++k) {
+            mini::wake *W = k ? W2 : W1;
+            size_t o0 = k ? n1 : 0;
+            for (size_t i = 0; i < W->_numParticles; ++i) {
+                for (int c = 0; c < 3; ++c) {
+                    pos[3 * (o0 + i) + c] = W->_position->data[3 * i + c];
+                    vort[3 * (o0 + i) + c] = W->_vorticity->data[3 * i + c];
+                }
+                rad[o0 + i] = W->_radius->data[i];
+                vol[o0 + i] = W->_volume->data[i];
+            }
+        }
+        std::vector<double> vel(3 * n), dv(3 * n);
+        orc_rates(ORC_MODE_OMP_CLEAN, ORC_KERNEL_HIGHORDER, 1e-3, 2, wn, pos.data(), vort.data(),
+                  rad.data(), vol.data(), nullptr, vel.data(), dv.data());
+        // solve(): rates land in each wake's _velocity / _retvorcity
+        S.solve();
+        std::vector<double> gv(W1->_velocity->data, W1->_velocity->data + 3 * n1),
+            gd(W1->_retvorcity->data, W1->_retvorcity->data + 3 * n1);
+        gv.insert(gv.end(), W2->_velocity->data, W2->_velocity->data + 3 * n2);
+        gd.insert(gd.end(), W2->_retvorcity->data, W2->_retvorcity->data + 3 * n2);
+        double ev = relerr(gv, vel), ed = relerr(gd, dv);
+        std::printf("driver: solve() vs oracle: velocity %.3e vorticity rate %.3e\n", ev, ed);
+        bad |= !(ev < 1e-10 && ed < 1e-10);
+        // getStates / getRates layout
+        mini::vec st(S._size), rt(S._size), ref(S._size);
+        S.getStates(&st);
+        orc_pack_state(2, wn, pos.data(), vort.data(), ref.data);
+        bool same = true;
+        for (size_t i = 0; i < S._size; ++i) same = same && st.data[i] == ref.data[i];
+        std::printf("driver: getStates layout %s\n", same ? "exact" : "WRONG");
+        bad |= !same;
+        // __rk4(t, n).integrate(S, IO): 3 steps of dt = 0.01, a frame per step
+        gpu_rk4 IN(0.03, 3);
+        IN.integrate(&S, path);
+        std::vector<double> x0(ref.data, ref.data + S._size), xg(S._size);
+        mini::vec fin(S._size);
+        S.getStates(&fin);
+        xg.assign(fin.data, fin.data + S._size);
+        orc_rk4(ORC_MODE_OMP_CLEAN, ORC_KERNEL_HIGHORDER, 1e-3, 2, wn, rad.data(), vol.data(),
+                nullptr, x0.data(), 0.01, 3);
+        double ex = relerr(xg, x0);
+        std::printf("driver: integrate() 3 RK4 steps vs oracle: %.3e\n", ex);
+        bad |= !(ex < 1e-12);
+        // the last frame of the .wake file equals the final state
+        FILE *f = std::fopen(path, "rb");
+        std::fseek(f, 0, SEEK_END);
+        long sz = std::ftell(f);
+        long frame = 8 + 8 + 2 * 8 + 8 * (7 * n);
+        std::printf("driver: .wake file %ld bytes = 4 frames of %ld\n", sz, frame);
+        bad |= sz != 4 * frame;
+        std::fclose(f);
+        double om[3], im[3], ke;
+        S.device_diagnostics(om, im, &ke);
+        std::printf("driver: kinetic energy %.12e\n", ke);
+    } catch (const std::exception &e) {
+        std::printf("driver: exception %s\n", e.what());
+        bad = 1;
+    }
+    delete W1;
+    delete W2;
+    std::printf("driver: %s\n", bad ? "FAILED" : "passed");
+    return bad;
+}

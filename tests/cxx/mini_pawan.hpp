@@ -1,0 +1,125 @@
+// mini_pawan.hpp -- test double of pawan's class API (namespace pawan: __system,
+// __wake, __interaction, __integration, __rk4) with plain vector/matrix structs
+// in place of gsl_vector/gsl_matrix, so that the drop-in adapter
+// include/pawan_gpu.hpp can be compiled and run here.  It reproduces the
+// reference's observable behaviour that the adapter relies on: the state
+// layout (wake.cpp:87-120), addWake bookkeeping (interaction.cpp:35-39), the
+// .wake frame format (interaction.cpp:197-202, wake.cpp:68-73) and the
+// integrate() loop (integration.cpp:16-35).  Its own CPU solve() is unused.
+#pragma once
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+namespace mini {
+
+struct vec {
+    size_t size = 0;
+    double *data = nullptr;
+    explicit vec(size_t n) : size(n), data(new double[n]()) {}
+    ~vec() { delete[] data; }
+};
+struct mat {
+    size_t size1 = 0, size2 = 3, tda = 3;
+    double *data = nullptr;
+    explicit mat(size_t n) : size1(n), data(new double[3 * n]()) {}
+    ~mat() { delete[] data; }
+};
+
+class system {
+  public:
+    size_t _size = 0;
+    virtual ~system() = default;
+    virtual void solve() {}
+    virtual void write(FILE *) {}
+    virtual void setStates(const vec *) {}
+    virtual void getRates(vec *) {}
+    virtual void getStates(vec *) {}
+};
+
+class wake {
+  public:
+    size_t _size, _numParticles;
+    mat *_position, *_velocity, *_vorticity, *_retvorcity;
+    vec *_radius, *_volume;
+    explicit wake(size_t n)
+        : _size(6 * n), _numParticles(n), _position(new mat(n)), _velocity(new mat(n)),
+          _vorticity(new mat(n)), _retvorcity(new mat(n)), _radius(new vec(n)), _volume(new vec(n)) {}
+    virtual ~wake() {
+        delete _position; delete _velocity; delete _vorticity; delete _retvorcity;
+        delete _radius; delete _volume;
+    }
+    void write(FILE *f) {  // wake.cpp:68-73
+        fwrite(&_numParticles, sizeof(size_t), 1, f);
+        fwrite(_position->data, sizeof(double), 3 * _numParticles, f);
+        fwrite(_vorticity->data, sizeof(double), 3 * _numParticles, f);
+        fwrite(_radius->data, sizeof(double), _numParticles, f);
+    }
+    void setStates(const double *s) {  // wake.cpp:87-96
+        std::memcpy(_position->data, s, 3 * _numParticles * sizeof(double));
+        std::memcpy(_vorticity->data, s + 3 * _numParticles, 3 * _numParticles * sizeof(double));
+    }
+};
+
+class interaction : public system {
+  protected:
+    double _nu = 2.5e-3;
+    size_t _nWake = 0;
+
+  private:
+    std::vector<wake *> _W;
+
+  public:
+    interaction() = default;
+    virtual void addWake(wake *W) {
+        _W.push_back(W);
+        _size += W->_size;
+        _nWake++;
+    }
+    virtual void diagnose() {}
+    void write(FILE *f) override {
+        fwrite(&_nWake, sizeof(size_t), 1, f);
+        for (auto *w : _W) w->write(f);
+    }
+    void setStates(const vec *s) override {
+        size_t off = 0;
+        for (auto *w : _W) {
+            w->setStates(s->data + off);
+            off += w->_size;
+        }
+    }
+};
+
+class integration {
+  protected:
+    double _dt, _t;
+    size_t _n;
+    virtual void step(const double &, system *, vec *) {}
+
+  public:
+    integration(const double &t, const size_t &n) : _dt(t / n), _t(t), _n(n) {}
+    virtual ~integration() = default;
+    // integration.cpp:16-35 with a file name in place of __io
+    void integrate(system *S, const char *path) {
+        vec states(S->_size);
+        FILE *f = std::fopen(path, "wb");
+        double t = 0.0;
+        std::fwrite(&t, sizeof(double), 1, f);
+        S->write(f);
+        S->getStates(&states);
+        for (size_t i = 1; i <= _n; ++i) {
+            t = i * _dt;
+            step(_dt, S, &states);
+            std::fwrite(&t, sizeof(double), 1, f);
+            S->write(f);
+        }
+        std::fclose(f);
+    }
+};
+
+class rk4 : public integration {
+  public:
+    rk4(const double &t, const size_t &n) : integration(t, n) {}
+};
+
+}  // namespace mini
