@@ -8,11 +8,12 @@
 //   using __gpu_rk4 = pawan_gpu::rk4<pawan::__rk4, pawan::__system, gsl_vector>;
 //
 // Requirements on the template arguments (all met by pawan's classes):
-//   Interaction  virtual addWake(Wake*), solve(), write(FILE*), diagnose(),
+//   Interaction  virtual addWake(Wake*), solve(), resolve(), write(FILE*), diagnose(),
 //                setStates(const Vector*), getRates(Vector*), getStates(Vector*)
 //                (interaction.h:181-234); protected double _nu (interaction.h:24)
 //   Wake         _numParticles; _position, _vorticity, _velocity, _retvorcity
-//                with ->data holding N x 3 row-major doubles (tda == 3);
+//                and _vorticityfield with ->data holding N x 3 row-major
+//                doubles (tda == 3);
 //                _radius, _volume with ->data holding N doubles (wake.h:33-42)
 //   Vector       ->data holding the state/rate vector (gsl_vector, stride 1)
 //   RK4          RK4(const double &t, const size_t &n); protected virtual
@@ -108,6 +109,19 @@ class interaction : public Interaction, public device_system {
     // interaction.cpp:213-229
     void getRates(Vector *rate) override { check(pw_get_rates(ctx_, rate->data), ctx_); }
     void getStates(Vector *state) override { check(pw_get_state(ctx_, state->data), ctx_); }
+
+    // interaction.cpp:45-47 -> influence(): every wake's _vorticityfield
+    void resolve() override {
+        size_t total = 0;
+        for (Wake *W : wakes_) total += W->_numParticles;
+        std::vector<double> field(3 * total);
+        check(pw_resolve(ctx_, field.data()), ctx_);
+        size_t o = 0;
+        for (Wake *W : wakes_) {
+            for (size_t k = 0; k < 3 * W->_numParticles; ++k) W->_vorticityfield->data[k] = field[o + k];
+            o += 3 * W->_numParticles;
+        }
+    }
 
     // interaction.cpp:197-202: frames carry the device state
     void write(FILE *f) override {

@@ -32,6 +32,7 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -109,9 +110,41 @@ int pw_rk4_steps(pw_ctx *ctx, double dt, size_t nsteps);
 int pw_euler_steps(pw_ctx *ctx, double dt, size_t nsteps);
 
 /* calculateTotalVorticity / calculateLinearImpulse (interaction.cpp:240-259)
- * and calculateKineticEnergy (interaction.cpp:351-383) summed over wakes
- * and wake pairs; any pointer may be NULL to skip that quantity */
+ * and calculateKineticEnergy (interaction.cpp:351-366) over all particles;
+ * any pointer may be NULL to skip that quantity (ke == NULL skips the O(N^2)
+ * pass) */
 int pw_diagnostics(pw_ctx *ctx, double omega[3], double impulse[3], double *ke);
+
+/* Everything __interaction::diagnose() reports (interaction.cpp:49-93), plus
+ * the kinetic energy it leaves out: total vorticity, linear and angular
+ * impulse (:240-275), helicity (:317-349), enstrophy (:277-315), kinetic
+ * energy (:351-366), over all particles. */
+typedef struct pw_diag {
+    double total_vorticity[3];
+    double linear_impulse[3];
+    double angular_impulse[3];
+    double helicity;
+    double enstrophy;
+    double kinetic_energy;
+} pw_diag;
+int pw_diagnose(pw_ctx *ctx, pw_diag *out);
+
+/* __interaction::resolve -> influence() (interaction.cpp:45-47,110-155):
+ * the vorticity field at every particle, N x 3 with wakes in order
+ * (each wake's __wake::_vorticityfield) */
+int pw_resolve(pw_ctx *ctx, double *field);
+
+/* __interaction::write (interaction.cpp:197-202): nWake (size_t), then per
+ * wake N (size_t), positions N x 3, vorticities N x 3, radii N, all binary64
+ * (wake.cpp:68-73); integrate() writes the time before each frame */
+int pw_write(pw_ctx *ctx, FILE *f);
+
+/* Particle emission: append n particles to wake `wake` (its _numParticles
+ * grows; the state vector grows by 6n).  Device state of the particles
+ * already present is kept.  No reference counterpart (__wake has a fixed
+ * particle count); the growing rotor-wake workload needs it. */
+int pw_append_particles(pw_ctx *ctx, size_t wake, size_t n, const double *pos, const double *vort,
+                        const double *radius, const double *volume);
 
 /* device time of the last pw_solve / pw_rk4_steps / pw_euler_steps call,
  * measured with HIP events on the context's stream: total ms and the summed

@@ -31,6 +31,12 @@ class PwJobSpec(C.Structure):
                 ("uinf", C.c_double * 3), ("precision", C.c_int)]
 
 
+class PwDiag(C.Structure):
+    _fields_ = [("total_vorticity", C.c_double * 3), ("linear_impulse", C.c_double * 3),
+                ("angular_impulse", C.c_double * 3), ("helicity", C.c_double),
+                ("enstrophy", C.c_double), ("kinetic_energy", C.c_double)]
+
+
 class PwServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("finished", C.c_int64),
                 ("failed", C.c_int64), ("cancelled", C.c_int64), ("queued", C.c_int64),
@@ -61,6 +67,10 @@ SIGNATURES = {
     "pw_euler_steps": (C.c_int, [C.c_void_p, C.c_double, C.c_size_t]),
     "pw_diagnostics": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "pw_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "pw_diagnose": (C.c_int, [C.c_void_p, C.POINTER(PwDiag)]),
+    "pw_resolve": (C.c_int, [C.c_void_p, _dp]),
+    "pw_write": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pw_append_particles": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, _dp, _dp, _dp, _dp]),
     "pw_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int]),
     "pw_batch_destroy": (C.c_int, [C.c_void_p]),
     "pw_batch_last_error": (C.c_char_p, [C.c_void_p]),

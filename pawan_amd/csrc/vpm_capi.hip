@@ -276,7 +276,86 @@ int pw_diagnostics(pw_ctx *ctx, double omega[3], double impulse[3], double *ke) 
     if (!ctx) return PW_EINVAL;
     return guarded(&ctx->err, [&] {
         ctx->ready();
-        ctx->eng->diagnostics(ctx->job, omega, impulse, ke);
+        pw::Diagnostics d;
+        ctx->eng->diagnostics(ctx->job, &d, ke != nullptr);
+        if (omega) std::memcpy(omega, d.omega, sizeof d.omega);
+        if (impulse) std::memcpy(impulse, d.impulse, sizeof d.impulse);
+        if (ke) *ke = d.ke;
+    });
+}
+
+int pw_diagnose(pw_ctx *ctx, pw_diag *out) {
+    if (!ctx || !out) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        ctx->ready();
+        pw::Diagnostics d;
+        ctx->eng->diagnostics(ctx->job, &d, true);
+        std::memcpy(out->total_vorticity, d.omega, sizeof d.omega);
+        std::memcpy(out->linear_impulse, d.impulse, sizeof d.impulse);
+        std::memcpy(out->angular_impulse, d.angular, sizeof d.angular);
+        out->helicity = d.helicity;
+        out->enstrophy = d.enstrophy;
+        out->kinetic_energy = d.ke;
+    });
+}
+
+int pw_resolve(pw_ctx *ctx, double *field) {
+    if (!ctx || !field) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        ctx->ready();
+        ctx->eng->influence(ctx->job, field);
+    });
+}
+
+int pw_write(pw_ctx *ctx, FILE *f) {
+    if (!ctx || !f) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        ctx->ready();
+        const size_t n = ctx->total();
+        std::vector<double> pos(3 * n), vort(3 * n);
+        ctx->eng->download_aos(ctx->job, 0, pos.data());
+        ctx->eng->download_aos(ctx->job, 1, vort.data());
+        size_t nw = ctx->wake_n.size();
+        bool ok = std::fwrite(&nw, sizeof(size_t), 1, f) == 1;
+        for (size_t w = 0; w < nw && ok; ++w) {
+            const size_t m = ctx->wake_n[w], o = (size_t)ctx->woff[w];
+            ok = std::fwrite(&m, sizeof(size_t), 1, f) == 1 &&
+                 std::fwrite(pos.data() + 3 * o, sizeof(double), 3 * m, f) == 3 * m &&
+                 std::fwrite(vort.data() + 3 * o, sizeof(double), 3 * m, f) == 3 * m &&
+                 std::fwrite(ctx->rad.data() + o, sizeof(double), m, f) == m;
+        }
+        if (!ok) throw Error(PW_EINVAL, "write to the file failed");
+    });
+}
+
+int pw_append_particles(pw_ctx *ctx, size_t wake, size_t n, const double *pos, const double *vort,
+                        const double *radius, const double *volume) {
+    if (!ctx) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        if (wake >= ctx->wake_n.size()) throw Error(PW_EINVAL, "no such wake");
+        if (n == 0) return;
+        if (!pos || !vort || !radius || !volume) throw Error(PW_EINVAL, "null array");
+        const size_t last = ctx->wake_n.size() - 1;
+        if (ctx->job && !ctx->dirty && wake == last) {
+            // the wake's particles end the device arrays: append in place
+            ctx->eng->append(ctx->job, n, pos, vort, radius, volume);
+            ctx->pos.insert(ctx->pos.end(), pos, pos + 3 * n);
+            ctx->vort.insert(ctx->vort.end(), vort, vort + 3 * n);
+        } else {
+            if (ctx->job && !ctx->dirty) {  // keep the device state, then insert
+                ctx->eng->download_aos(ctx->job, 0, ctx->pos.data());
+                ctx->eng->download_aos(ctx->job, 1, ctx->vort.data());
+            }
+            const size_t at = (size_t)ctx->woff[wake + 1];
+            ctx->pos.insert(ctx->pos.begin() + 3 * at, pos, pos + 3 * n);
+            ctx->vort.insert(ctx->vort.begin() + 3 * at, vort, vort + 3 * n);
+            ctx->dirty = true;
+        }
+        const size_t at = (size_t)ctx->woff[wake + 1];
+        ctx->rad.insert(ctx->rad.begin() + at, radius, radius + n);
+        ctx->vol.insert(ctx->vol.begin() + at, volume, volume + n);
+        ctx->wake_n[wake] += n;
+        for (size_t w = wake + 1; w < ctx->woff.size(); ++w) ctx->woff[w] += (int64_t)n;
     });
 }
 

@@ -520,12 +520,13 @@ __global__ __launch_bounds__(256) void aos_soa_kernel(double *__restrict__ aos,
 }
 
 // ------------------------------------------------------------- diagnostics
-// Per 256-particle tile: sum a (3) and sum x x a (3), written to part6[tile].
+// Per 256-particle tile: sum a (3), sum x x a (3) and sum x x (x x a) (3),
+// written to out[tile][9] (interaction.cpp:240-275 before scaling).
 __global__ __launch_bounds__(256) void linear_diag_kernel(const double *__restrict__ xs, int n,
                                                           int npad, double *__restrict__ out) {
-    __shared__ double red[6][256];
+    __shared__ double red[9][256];
     const int i = blockIdx.x * 256 + threadIdx.x;
-    double v[6] = {0, 0, 0, 0, 0, 0};
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (i < n) {
         const double x = xs[i], y = xs[npad + i], z = xs[2 * (size_t)npad + i];
         const double ax = xs[3 * (size_t)npad + i], ay = xs[4 * (size_t)npad + i],
@@ -536,36 +537,51 @@ __global__ __launch_bounds__(256) void linear_diag_kernel(const double *__restri
         v[3] = y * az - z * ay;
         v[4] = z * ax - x * az;
         v[5] = x * ay - y * ax;
+        v[6] = y * v[5] - z * v[4];
+        v[7] = z * v[3] - x * v[5];
+        v[8] = x * v[4] - y * v[3];
     }
-    for (int m = 0; m < 6; ++m) red[m][threadIdx.x] = v[m];
+    for (int m = 0; m < 9; ++m) red[m][threadIdx.x] = v[m];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)
-            for (int m = 0; m < 6; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + w];
+            for (int m = 0; m < 9; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x < 6) out[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 9) out[blockIdx.x * 9 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Kinetic energy pair sums (interaction_utils.h:197-240):
-// e_i = a_i.a_i/(8 pi s_i) + sum_{j != i} ((rho^2 + 2 sig^2)(a_i.a_j) + (d.a_i)(d.a_j))
-//                                        / (16 pi (rho^2 + sig^2)^1.5)
-// so that KE = 2 sum_i e_i.  Grid: (target tiles, source chunks); out is
-// [nchunk][npad], chunk 0 carries the self term.
-__global__ __launch_bounds__(256) void ke_pair_kernel(const double *__restrict__ xs,
-                                                      const double *__restrict__ rad, int n,
-                                                      int npad, int chunk_len,
-                                                      double *__restrict__ out) {
+// Quadratic diagnostics per target, summed over one source chunk
+// (blockIdx.y); out is [nchunk][3][npad]:
+//   0  kinetic energy   e_i = a_i.a_i/(8 pi s_i) + sum_{j!=i} ((r^2 + 2 sig^2)(a_i.a_j)
+//                             + (d.a_i)(d.a_j)) / (16 pi (r^2 + sig^2)^1.5)
+//                       (KINETICENERGY, interaction_utils.h:197-240); KE = 2 sum_i e_i
+//   1  enstrophy        (5/4 pi) a_i.a_i / s_i^3 + sum_{j!=i} F1 (a_i.a_j) + F2 (a_i.d)(a_j.d),
+//                       F1 = (10 sig^4 - 2 r^4 - 7 sig^2 r^2)/(8 pi (r^2+sig^2)^3.5),
+//                       F2 = -15 r^2/(8 pi (r^2+sig^2)^3.5)   (ENST, highorder.h:69-86;
+//                       ENSTROPHY, interaction_utils.h:107-156); zero for GAUSSIAN (gaussian.h:62-81)
+//   2  helicity         sum_{j!=i} q(r, sig) d.(a_i x a_j)   (HELICITY, interaction_utils.h:167-190)
+// with d = x_j - x_i, sig^2 = (s_i^2 + s_j^2)/2.  Chunk 0 carries the self terms.
+template <int KIND>
+__global__ __launch_bounds__(256) void diag_pair_kernel(const double *__restrict__ xs,
+                                                        const double *__restrict__ rad, int n,
+                                                        int npad, int chunk_len,
+                                                        double *__restrict__ out) {
     __shared__ double sh[7][256];
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int ii = i < n ? i : n - 1;
     const double x = xs[ii], y = xs[npad + ii], z = xs[2 * (size_t)npad + ii];
     const double ax = xs[3 * (size_t)npad + ii], ay = xs[4 * (size_t)npad + ii],
                  az = xs[5 * (size_t)npad + ii];
-    const double hs = 0.5 * rad[ii] * rad[ii];
+    const double si = rad[ii];
+    const double hs = 0.5 * si * si;
     const int c0 = blockIdx.y * chunk_len, c1 = min(c0 + chunk_len, n);
-    double e = 0.0;
-    if (blockIdx.y == 0) e = (ax * ax + ay * ay + az * az) / (8.0 * K_PI * rad[ii]);
+    double e = 0.0, en = 0.0, he = 0.0;
+    if (blockIdx.y == 0) {
+        const double a2 = ax * ax + ay * ay + az * az;
+        e = a2 / (8.0 * K_PI * si);
+        if (KIND == KIND_HIGHORDER) en = 5.0 / (4.0 * K_PI) * a2 / (si * si * si);
+    }
     for (int s0 = c0; s0 < c1; s0 += 256) {
         const int cnt = min(256, c1 - s0);
         __syncthreads();
@@ -578,16 +594,102 @@ __global__ __launch_bounds__(256) void ke_pair_kernel(const double *__restrict__
         for (int k = 0; k < cnt; ++k) {
             if (s0 + k == i) continue;
             const double dx = sh[0][k] - x, dy = sh[1][k] - y, dz = sh[2][k] - z;
+            const double bx = sh[3][k], by = sh[4][k], bz = sh[5][k];
             const double r2 = dx * dx + dy * dy + dz * dz;
             const double sg2 = hs + sh[6][k];
-            const double aa = ax * sh[3][k] + ay * sh[4][k] + az * sh[5][k];
+            const double aa = ax * bx + ay * by + az * bz;
             const double da = dx * ax + dy * ay + dz * az;
-            const double db = dx * sh[3][k] + dy * sh[4][k] + dz * sh[5][k];
+            const double db = dx * bx + dy * by + dz * bz;
             const double w = rsq_nr(r2 + sg2);
-            e += ((r2 + 2.0 * sg2) * aa + da * db) * (w * w * w) * (1.0 / (16.0 * K_PI));
+            const double w3 = w * w * w;
+            e += ((r2 + 2.0 * sg2) * aa + da * db) * w3 * (1.0 / (16.0 * K_PI));
+            if (KIND == KIND_HIGHORDER) {
+                const double f = w3 * w3 * w * (1.0 / (8.0 * K_PI));
+                en += f * ((10.0 * sg2 * sg2 - 2.0 * r2 * r2 - 7.0 * sg2 * r2) * aa -
+                           15.0 * r2 * da * db);
+            }
+            const double isg = rsq_nr(sg2);
+            const double is2 = isg * isg;
+            const double t = r2 * is2;
+            double Q, Fp, Zs;
+            kernel64<KIND>(t, rsq_nr(t), Q, Fp, Zs);
+            const double ex = ay * bz - az * by, ey = az * bx - ax * bz, ez = ax * by - ay * bx;
+            he += Q * (isg * is2) * (dx * ex + dy * ey + dz * ez);
         }
     }
-    if (i < n) out[(size_t)blockIdx.y * npad + i] = e;
+    if (i < n) {
+        double *o = out + (size_t)blockIdx.y * 3 * npad + i;
+        o[0] = e;
+        o[npad] = en;
+        o[2 * (size_t)npad] = he;
+    }
+}
+
+// Vorticity field (influence(), interaction.cpp:110-155): for every target
+//   k_i = sum_j Z(rho_ij, sigma_ij) a_j,  sigma_ij^2 = (s_i^2 + s_j^2)/2,
+// the j = i term being SELFINFLUENCE's Z(0, s_i) a_i (interaction_utils.h:592-605).
+// out is [nchunk][3][npad].
+template <int KIND>
+__global__ __launch_bounds__(256) void influence_kernel(const double *__restrict__ xs,
+                                                        const double *__restrict__ rad, int n,
+                                                        int npad, int chunk_len,
+                                                        double *__restrict__ out) {
+    __shared__ double sh[7][256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int ii = i < n ? i : n - 1;
+    const double x = xs[ii], y = xs[npad + ii], z = xs[2 * (size_t)npad + ii];
+    const double hs = 0.5 * rad[ii] * rad[ii];
+    const int c0 = blockIdx.y * chunk_len, c1 = min(c0 + chunk_len, n);
+    double kx = 0.0, ky = 0.0, kz = 0.0;
+    for (int s0 = c0; s0 < c1; s0 += 256) {
+        const int cnt = min(256, c1 - s0);
+        __syncthreads();
+        const int j = s0 + (int)threadIdx.x;
+        if ((int)threadIdx.x < cnt) {
+            for (int m = 0; m < 6; ++m) sh[m][threadIdx.x] = xs[(size_t)m * npad + j];
+            sh[6][threadIdx.x] = 0.5 * rad[j] * rad[j];
+        }
+        __syncthreads();
+        for (int k = 0; k < cnt; ++k) {
+            const double dx = sh[0][k] - x, dy = sh[1][k] - y, dz = sh[2][k] - z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double sg2 = hs + sh[6][k];
+            const double isg = rsq_nr(sg2);
+            const double is2 = isg * isg;
+            const double t = r2 * is2;
+            double Zs;
+            if (KIND == KIND_HIGHORDER) {
+                const double w = rsq_nr(t + 1.0);
+                const double iu = w * w, iu2 = iu * iu, iu4 = iu2 * iu2;
+                Zs = K_CZ * (iu4 * iu2 * iu * w);
+            } else {
+                Zs = K_CG * exp(-0.5 * t);
+            }
+            const double Z = Zs * isg * is2;
+            kx += Z * sh[3][k];
+            ky += Z * sh[4][k];
+            kz += Z * sh[5][k];
+        }
+    }
+    if (i < n) {
+        double *o = out + (size_t)blockIdx.y * 3 * npad + i;
+        o[0] = kx;
+        o[npad] = ky;
+        o[2 * (size_t)npad] = kz;
+    }
+}
+
+// out[m][i] = sum over chunks c (in order) of in[c][m][i], m < ncomp, i < n.
+__global__ __launch_bounds__(256) void chunk_sum_kernel(const double *__restrict__ in,
+                                                        double *__restrict__ out, int nchunk,
+                                                        int ncomp, int n, int npad) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    for (int m = 0; m < ncomp; ++m) {
+        double s = 0.0;
+        for (int c = 0; c < nchunk; ++c) s += in[((size_t)c * ncomp + m) * npad + i];
+        out[(size_t)m * npad + i] = s;
+    }
 }
 
 }  // namespace pw

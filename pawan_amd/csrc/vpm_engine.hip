@@ -57,6 +57,7 @@ void HostBuf::ensure(size_t b) {
 }
 
 static size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+static int grid_for(size_t n) { return (int)std::min<size_t>(4096, (n + 255) / 256 + 1); }
 
 size_t chunk_for(size_t n, int TB) {
     if (n == 0) return (size_t)TB;
@@ -206,6 +207,8 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     j->chunk_len = (int)L;
     j->nchunk = (int)((n + L - 1) / L);
     j->ntile = (int)((n + tb_ - 1) / tb_);
+    j->chunk_override = chunk_override;
+    j->part_chunks = j->nchunk;
     j->uniform = true;
     for (size_t i = 1; i < n; ++i)
         if (std::memcmp(&radius[i], &radius[0], sizeof(double)) != 0) {
@@ -268,6 +271,109 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     plan_dirty_ = true;
     upload_state_soa(raw, pos, vort);
     return raw;
+}
+
+static void fill_records(void *dst, int precision, size_t m, const double *pos, const double *vort,
+                         const double *radius, const double *volume) {
+    for (size_t i = 0; i < m; ++i) {
+        if (precision == PREC_FP64) {
+            Rec64 &r = ((Rec64 *)dst)[i];
+            r.x = pos[3 * i];
+            r.y = pos[3 * i + 1];
+            r.z = pos[3 * i + 2];
+            r.h = 0.5 * radius[i] * radius[i];
+            r.ax = vort[3 * i];
+            r.ay = vort[3 * i + 1];
+            r.az = vort[3 * i + 2];
+            r.v = volume[i];
+        } else {
+            Rec32 &r = ((Rec32 *)dst)[i];
+            r.x = pos[3 * i];
+            r.y = pos[3 * i + 1];
+            r.z = pos[3 * i + 2];
+            r.h = 0.5 * radius[i] * radius[i];
+            r.ax = (float)vort[3 * i];
+            r.ay = (float)vort[3 * i + 1];
+            r.az = (float)vort[3 * i + 2];
+            r.v = (float)volume[i];
+        }
+    }
+}
+
+void Engine::append(Job *j, size_t m, const double *pos, const double *vort, const double *radius,
+                    const double *volume) {
+    if (m == 0) return;
+    if (!pos || !vort || !radius || !volume) throw Error(PW_EINVAL, "null array");
+    const size_t n = j->n, n2 = n + m;
+    if (n2 > (size_t)INT32_MAX / 2) throw Error(PW_EINVAL, "too many particles");
+    PW_HIP(hipSetDevice(device_));
+    PW_HIP(hipStreamSynchronize(stream_));
+    const size_t rb = rec_bytes(j->par.precision);
+    if (n2 > j->npad) {  // grow: new stride, keep the device contents
+        const size_t np = j->npad, np2 = round_up(std::max(2 * np, n2), 256);
+        DevBuf rec(np2 * rb), soa(5 * 6 * np2 * sizeof(double)), vol(2 * np2 * sizeof(double));
+        PW_HIP(hipMemsetAsync(rec.p, 0, rec.bytes, stream_));
+        PW_HIP(hipMemsetAsync(soa.p, 0, soa.bytes, stream_));
+        PW_HIP(hipMemsetAsync(vol.p, 0, vol.bytes, stream_));
+        PW_HIP(hipMemcpyAsync(rec.p, j->rec.p, n * rb, hipMemcpyDeviceToDevice, stream_));
+        for (size_t b = 0; b < 30; ++b)
+            PW_HIP(hipMemcpyAsync(soa.as<double>() + b * np2, j->soa.as<double>() + b * np,
+                                  n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        for (size_t b = 0; b < 2; ++b)
+            PW_HIP(hipMemcpyAsync(vol.as<double>() + b * np2, j->vol.as<double>() + b * np,
+                                  n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        PW_HIP(hipStreamSynchronize(stream_));
+        std::swap(j->rec.p, rec.p);
+        std::swap(j->rec.bytes, rec.bytes);
+        std::swap(j->soa.p, soa.p);
+        std::swap(j->soa.bytes, soa.bytes);
+        std::swap(j->vol.p, vol.p);
+        std::swap(j->vol.bytes, vol.bytes);
+        j->npad = np2;
+        j->part_chunks = 0;  // partial sums are scratch: reallocated below
+    }
+    const size_t np = j->npad;
+    // new particles' records, state (x0 = xs between steps), volume, radius
+    stage_.ensure(std::max(m * rb, 8 * m * sizeof(double)));
+    fill_records(stage_.p, j->par.precision, m, pos, vort, radius, volume);
+    PW_HIP(hipMemcpyAsync(j->rec.as<char>() + n * rb, stage_.p, m * rb, hipMemcpyHostToDevice,
+                          stream_));
+    PW_HIP(hipStreamSynchronize(stream_));
+    double *h = stage_.as<double>();
+    for (size_t i = 0; i < m; ++i)
+        for (int c = 0; c < 3; ++c) {
+            h[c * m + i] = pos[3 * i + c];
+            h[(3 + c) * m + i] = vort[3 * i + c];
+        }
+    for (size_t i = 0; i < m; ++i) {
+        h[6 * m + i] = volume[i];
+        h[7 * m + i] = radius[i];
+    }
+    double *soa = j->soa.as<double>();
+    for (int c = 0; c < 6; ++c) {
+        PW_HIP(hipMemcpyAsync(soa + c * np + n, h + c * m, m * sizeof(double),
+                              hipMemcpyHostToDevice, stream_));  // x0
+        PW_HIP(hipMemcpyAsync(soa + (6 + c) * np + n, h + c * m, m * sizeof(double),
+                              hipMemcpyHostToDevice, stream_));  // xs
+    }
+    PW_HIP(hipMemcpyAsync(j->vol.as<double>() + n, h + 6 * m, m * sizeof(double),
+                          hipMemcpyHostToDevice, stream_));
+    PW_HIP(hipMemcpyAsync(j->vol.as<double>() + np + n, h + 7 * m, m * sizeof(double),
+                          hipMemcpyHostToDevice, stream_));
+    PW_HIP(hipStreamSynchronize(stream_));
+    for (size_t i = 0; i < m; ++i)
+        if (std::memcmp(&radius[i], &j->sigma, sizeof(double)) != 0) j->uniform = false;
+    j->n = n2;
+    const size_t L = j->chunk_override ? j->chunk_override : chunk_for(n2, tb_);
+    j->chunk_len = (int)L;
+    j->nchunk = (int)((n2 + L - 1) / L);
+    j->ntile = (int)((n2 + tb_ - 1) / tb_);
+    if (j->nchunk > j->part_chunks) {
+        j->part_chunks = std::max(j->nchunk, 16);
+        j->part.reset((size_t)j->part_chunks * NCOMP * np * sizeof(double));
+    }
+    j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + j->flag.bytes;
+    plan_dirty_ = true;
 }
 
 void Engine::remove_job(int64_t id) {
@@ -484,7 +590,6 @@ void Engine::euler(size_t nsteps) {
     last_pairs = (double)nsteps * pairs_per_rhs();
 }
 
-static int grid_for(size_t n) { return (int)std::min<size_t>(4096, (n + 255) / 256 + 1); }
 
 void Engine::upload_state_soa(Job *j, const double *pos, const double *vort) {
     PW_HIP(hipSetDevice(device_));
@@ -570,47 +675,90 @@ void Engine::download_state_layout(Job *j, int which, double *state,
     PW_HIP(hipStreamSynchronize(stream_));
 }
 
-void Engine::diagnostics(Job *j, double omega[3], double impulse[3], double *ke) {
+void Engine::diagnostics(Job *j, Diagnostics *out, bool quadratic) {
     PW_HIP(hipSetDevice(device_));
     const size_t np = j->npad, n = j->n;
     JobDev d = j->dev();
     const int nt = (int)((n + 255) / 256);
-    if (omega || impulse) {
-        DevBuf out(nt * 6 * sizeof(double));
+    {
+        DevBuf buf(nt * 9 * sizeof(double));
         hipLaunchKernelGGL(linear_diag_kernel, dim3(nt), dim3(256), 0, stream_, (const double *)d.x0,
-                           (int)n, (int)np, out.as<double>());
+                           (int)n, (int)np, buf.as<double>());
         PW_HIP(hipGetLastError());
-        std::vector<double> h(nt * 6);
-        PW_HIP(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+        std::vector<double> h(nt * 9);
+        PW_HIP(hipMemcpyAsync(h.data(), buf.p, h.size() * sizeof(double), hipMemcpyDeviceToHost,
                               stream_));
         PW_HIP(hipStreamSynchronize(stream_));
-        double s[6] = {0, 0, 0, 0, 0, 0};
+        double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int t = 0; t < nt; ++t)
-            for (int m = 0; m < 6; ++m) s[m] += h[t * 6 + m];
-        if (omega)
-            for (int c = 0; c < 3; ++c) omega[c] = s[c];
-        if (impulse)
-            for (int c = 0; c < 3; ++c) impulse[c] = 0.5 * s[3 + c];
-    }
-    if (ke) {
-        const int L = (int)chunk_for(n, 256);
-        const int nc = (int)((n + L - 1) / L);
-        DevBuf out((size_t)nc * np * sizeof(double));
-        hipLaunchKernelGGL(ke_pair_kernel, dim3(nt, nc), dim3(256), 0, stream_, (const double *)d.x0,
-                           (const double *)(d.vol + np), (int)n, (int)np, L, out.as<double>());
-        PW_HIP(hipGetLastError());
-        std::vector<double> h((size_t)nc * np);
-        PW_HIP(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                              stream_));
-        PW_HIP(hipStreamSynchronize(stream_));
-        double e = 0.0;
-        for (size_t i = 0; i < n; ++i) {
-            double ei = 0.0;
-            for (int c = 0; c < nc; ++c) ei += h[(size_t)c * np + i];
-            e += ei;
+            for (int m = 0; m < 9; ++m) s[m] += h[t * 9 + m];
+        for (int c = 0; c < 3; ++c) {
+            out->omega[c] = s[c];
+            out->impulse[c] = s[3 + c] * 0.5;
+            out->angular[c] = s[6 + c] * (1.0 / 3.0);
         }
-        *ke = 2.0 * e;
     }
+    out->ke = out->enstrophy = out->helicity = 0.0;
+    if (!quadratic) return;
+    const int L = (int)chunk_for(n, 256);
+    const int nc = (int)((n + L - 1) / L);
+    DevBuf part((size_t)nc * 3 * np * sizeof(double)), sum(3 * np * sizeof(double));
+    if (kind_ == KIND_GAUSSIAN)
+        hipLaunchKernelGGL(diag_pair_kernel<KIND_GAUSSIAN>, dim3(nt, nc), dim3(256), 0, stream_,
+                           (const double *)d.x0, (const double *)(d.vol + np), (int)n, (int)np, L,
+                           part.as<double>());
+    else
+        hipLaunchKernelGGL(diag_pair_kernel<KIND_HIGHORDER>, dim3(nt, nc), dim3(256), 0, stream_,
+                           (const double *)d.x0, (const double *)(d.vol + np), (int)n, (int)np, L,
+                           part.as<double>());
+    PW_HIP(hipGetLastError());
+    hipLaunchKernelGGL(chunk_sum_kernel, dim3(nt), dim3(256), 0, stream_, (const double *)part.p,
+                       sum.as<double>(), nc, 3, (int)n, (int)np);
+    PW_HIP(hipGetLastError());
+    std::vector<double> h(3 * np);
+    PW_HIP(hipMemcpyAsync(h.data(), sum.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    PW_HIP(hipStreamSynchronize(stream_));
+    double e = 0.0, en = 0.0, he = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        e += h[i];
+        en += h[np + i];
+        he += h[2 * np + i];
+    }
+    out->ke = 2.0 * e;
+    out->enstrophy = en;
+    out->helicity = he;
+}
+
+void Engine::influence(Job *j, double *field_aos) {
+    PW_HIP(hipSetDevice(device_));
+    const size_t np = j->npad, n = j->n;
+    JobDev d = j->dev();
+    const int nt = (int)((n + 255) / 256);
+    const int L = (int)chunk_for(n, 256);
+    const int nc = (int)((n + L - 1) / L);
+    DevBuf part((size_t)nc * 3 * np * sizeof(double)), sum(3 * np * sizeof(double));
+    if (kind_ == KIND_GAUSSIAN)
+        hipLaunchKernelGGL(influence_kernel<KIND_GAUSSIAN>, dim3(nt, nc), dim3(256), 0, stream_,
+                           (const double *)d.x0, (const double *)(d.vol + np), (int)n, (int)np, L,
+                           part.as<double>());
+    else
+        hipLaunchKernelGGL(influence_kernel<KIND_HIGHORDER>, dim3(nt, nc), dim3(256), 0, stream_,
+                           (const double *)d.x0, (const double *)(d.vol + np), (int)n, (int)np, L,
+                           part.as<double>());
+    PW_HIP(hipGetLastError());
+    hipLaunchKernelGGL(chunk_sum_kernel, dim3(nt), dim3(256), 0, stream_, (const double *)part.p,
+                       sum.as<double>(), nc, 3, (int)n, (int)np);
+    PW_HIP(hipGetLastError());
+    if (dstage_.bytes < 3 * n * sizeof(double)) {
+        PW_HIP(hipStreamSynchronize(stream_));
+        dstage_.reset(round_up(6 * np * sizeof(double), 4096));
+    }
+    hipLaunchKernelGGL(aos_soa_kernel, dim3(grid_for(n)), dim3(256), 0, stream_,
+                       dstage_.as<double>(), sum.as<double>(), (int)n, (int)np, 1);
+    PW_HIP(hipGetLastError());
+    PW_HIP(hipMemcpyAsync(field_aos, dstage_.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost,
+                          stream_));
+    PW_HIP(hipStreamSynchronize(stream_));
 }
 
 bool Engine::take_flag(Job *j) {

@@ -64,6 +64,11 @@ size_t chunk_for(size_t n, int TB);
 // Bytes a resident job occupies on the device.
 size_t job_footprint(size_t n, int precision, int TB);
 
+struct Diagnostics {
+    double omega[3], impulse[3], angular[3];
+    double ke, enstrophy, helicity;
+};
+
 // One simulation resident on the device.
 struct Job {
     int64_t id = 0;
@@ -74,6 +79,8 @@ struct Job {
     double sigma = 0.0;
     DevBuf rec, soa, part, vol, flag;  // soa holds x0, xs, k1, k23, rate
     size_t bytes = 0;
+    size_t chunk_override = 0;  // fixed chunk length (0: policy)
+    int part_chunks = 0;        // chunks the partial-sum buffer holds
     JobDev dev() const;
 };
 
@@ -95,6 +102,10 @@ class Engine {
                  const double *radius, const double *volume, const JobParams &par,
                  size_t chunk_override = 0);
     void remove_job(int64_t id);
+    // Append m particles to a job (particle emission), between steps.  The
+    // device arrays grow geometrically; existing state is kept on the device.
+    void append(Job *j, size_t m, const double *pos, const double *vort, const double *radius,
+                const double *volume);
     Job *find(int64_t id);
     const std::vector<std::unique_ptr<Job>> &jobs() const { return jobs_; }
     void set_params(Job *j, const JobParams &par);
@@ -113,8 +124,11 @@ class Engine {
     void upload_state_layout(Job *j, const double *state, const std::vector<int64_t> &woff);
     void download_state_layout(Job *j, int which, double *state,
                                const std::vector<int64_t> &woff);  // which: 0 state 1 rates
-    // diagnostics of one job: sum a, 0.5 sum x x a, kinetic energy
-    void diagnostics(Job *j, double omega[3], double impulse[3], double *ke);
+    // diagnostics of one job (interaction.cpp:240-383); the quadratic ones
+    // (kinetic energy, enstrophy, helicity) are O(N^2) and optional
+    void diagnostics(Job *j, Diagnostics *out, bool quadratic);
+    // vorticity field at every particle (influence(), interaction.cpp:110-155), N x 3
+    void influence(Job *j, double *field_aos);
     // non-finite flags, read back and cleared
     bool take_flag(Job *j);
 

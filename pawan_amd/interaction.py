@@ -158,6 +158,61 @@ class Interaction:
         return {"total_vorticity": om, "linear_impulse": im,
                 "kinetic_energy": k.value if ke else None}
 
+    def diagnose_all(self):
+        """Everything diagnose() reports (interaction.cpp:49-93) plus kinetic
+        energy, computed on the device."""
+        d = _lib.PwDiag()
+        self._ck(lib().pw_diagnose(self._h, C.byref(d)))
+        return {"total_vorticity": np.array(d.total_vorticity[:]),
+                "linear_impulse": np.array(d.linear_impulse[:]),
+                "angular_impulse": np.array(d.angular_impulse[:]),
+                "helicity": d.helicity, "enstrophy": d.enstrophy,
+                "kinetic_energy": d.kinetic_energy}
+
+    # interaction.cpp:45-47 -> influence()
+    def resolve(self):
+        """Vorticity field at every particle (N x 3, wakes in order)."""
+        out = np.empty((self._size // 6, 3))
+        self._ck(lib().pw_resolve(self._h, _lib.ptr(out)))
+        return out
+
+    # interaction.cpp:197-202
+    def write(self, path, t=None, append=True):
+        """Append a frame to a .wake file: the time (when given, as
+        integrate() writes it) then nWake and each wake's particles."""
+        libc = C.CDLL(None)
+        libc.fopen.restype = C.c_void_p
+        libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+        libc.fclose.argtypes = [C.c_void_p]
+        f = libc.fopen(str(path).encode(), b"ab" if append else b"wb")
+        if not f:
+            raise OSError(f"cannot open {path}")
+        try:
+            if t is not None:
+                libc.fwrite.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p]
+                tt = C.c_double(t)
+                libc.fwrite(C.byref(tt), 8, 1, f)
+            self._ck(lib().pw_write(self._h, f))
+        finally:
+            libc.fclose(f)
+
+    def appendParticles(self, wake: int, pos, vort, radius, volume):
+        """Emit particles into wake ``wake`` (the state grows by 6 per particle)."""
+        pos, vort = _f64(pos).reshape(-1, 3), _f64(vort).reshape(-1, 3)
+        radius, volume = _f64(radius).reshape(-1), _f64(volume).reshape(-1)
+        self._ck(lib().pw_append_particles(self._h, wake, pos.shape[0], _lib.ptr(pos),
+                                           _lib.ptr(vort), _lib.ptr(radius), _lib.ptr(volume)))
+        W = self._W[wake]
+        W._position = np.concatenate([W._position, pos])
+        W._vorticity = np.concatenate([W._vorticity, vort])
+        W._radius = np.concatenate([W._radius, radius])
+        W._volume = np.concatenate([W._volume, volume])
+        W._numParticles = W._position.shape[0]
+        W._velocity = np.zeros_like(W._position)
+        W._retvorcity = np.zeros_like(W._position)
+        W._size = 6 * W._numParticles
+        self._size = lib().pw_size(self._h)
+
     def last_timing(self):
         t, p, n = C.c_double(), C.c_double(), C.c_double()
         dp = C.POINTER(C.c_double)

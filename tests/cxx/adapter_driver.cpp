@@ -82,6 +82,15 @@ int main(int argc, char **argv) {
         double ev = relerr(gv, vel), ed = relerr(gd, dv);
         std::printf("driver: solve() vs oracle: velocity %.3e vorticity rate %.3e\n", ev, ed);
         bad |= !(ev < 1e-10 && ed < 1e-10);
+        // resolve(): each wake's _vorticityfield
+        std::vector<double> fld(3 * n);
+        orc_influence(ORC_KERNEL_HIGHORDER, 2, wn, pos.data(), vort.data(), rad.data(), fld.data());
+        S.resolve();
+        std::vector<double> gf(W1->_vorticityfield->data, W1->_vorticityfield->data + 3 * n1);
+        gf.insert(gf.end(), W2->_vorticityfield->data, W2->_vorticityfield->data + 3 * n2);
+        double ef = relerr(gf, fld);
+        std::printf("driver: resolve() field vs oracle: %.3e\n", ef);
+        bad |= !(ef < 1e-12);
         // getStates / getRates layout
         mini::vec st(S._size), rt(S._size), ref(S._size);
         S.getStates(&st);

@@ -40,14 +40,15 @@ class system {
 class wake {
   public:
     size_t _size, _numParticles;
-    mat *_position, *_velocity, *_vorticity, *_retvorcity;
+    mat *_position, *_velocity, *_vorticity, *_retvorcity, *_vorticityfield;
     vec *_radius, *_volume;
     explicit wake(size_t n)
         : _size(6 * n), _numParticles(n), _position(new mat(n)), _velocity(new mat(n)),
-          _vorticity(new mat(n)), _retvorcity(new mat(n)), _radius(new vec(n)), _volume(new vec(n)) {}
+          _vorticity(new mat(n)), _retvorcity(new mat(n)), _vorticityfield(new mat(n)),
+          _radius(new vec(n)), _volume(new vec(n)) {}
     virtual ~wake() {
         delete _position; delete _velocity; delete _vorticity; delete _retvorcity;
-        delete _radius; delete _volume;
+        delete _vorticityfield; delete _radius; delete _volume;
     }
     void write(FILE *f) {  // wake.cpp:68-73
         fwrite(&_numParticles, sizeof(size_t), 1, f);
@@ -77,6 +78,7 @@ class interaction : public system {
         _nWake++;
     }
     virtual void diagnose() {}
+    virtual void resolve() {}
     void write(FILE *f) override {
         fwrite(&_nWake, sizeof(size_t), 1, f);
         for (auto *w : _W) w->write(f);

@@ -1,0 +1,146 @@
+"""Next rows of the path on the GPU: influence()/resolve(), the diagnostics
+diagnose() prints, the .wake frame writer and particle emission."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import pawan_amd as pa
+from pawan_amd import generators as gen
+from helpers import per_particle_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _two_wakes():
+    a = gen.vortex_ring(40, 16, R=1.0, core=0.1, seed=21)
+    b = gen.vortex_ring(30, 9, R=0.7, core=0.12, seed=22, center=(0.1, 0.0, 0.4))
+    w = {k: np.concatenate([a[k], b[k]]) for k in ("pos", "vort", "radius", "volume")}
+    return a, b, w
+
+
+@pytest.mark.parametrize("kernel,okind", [(pa.KERNEL_HIGHORDER, orc.HIGHORDER),
+                                          (pa.KERNEL_GAUSSIAN, orc.GAUSSIAN)])
+def test_resolve_influence(kernel, okind):
+    a, b, w = _two_wakes()
+    print(f"case influence kernel={kernel} N={a['n']}+{b['n']}", flush=True)
+    S = pa.Interaction(pa.Wake.from_dict(a), pa.Wake.from_dict(b), nu=1e-3, kernel=kernel)
+    f = S.resolve()
+    S.close()
+    ref = orc.influence(w["pos"], w["vort"], w["radius"], kind=okind, wake_n=[a["n"], b["n"]])
+    err = per_particle_rel(f, ref)
+    print(f"  field {err:.3e}", flush=True)
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("kernel,okind", [(pa.KERNEL_HIGHORDER, orc.HIGHORDER),
+                                          (pa.KERNEL_GAUSSIAN, orc.GAUSSIAN)])
+def test_diagnose_all(kernel, okind):
+    a, b, w = _two_wakes()
+    print(f"case diagnose kernel={kernel}", flush=True)
+    S = pa.Interaction(pa.Wake.from_dict(a), pa.Wake.from_dict(b), nu=1e-3, kernel=kernel)
+    d = S.diagnose_all()
+    S.close()
+    wn = [a["n"], b["n"]]
+    do = orc.diagnostics(w["pos"], w["vort"], w["radius"], wake_n=wn)
+    for k in ("total_vorticity", "linear_impulse", "angular_impulse"):
+        scale = np.abs(do[k]).max() + 1e-300
+        assert np.allclose(d[k], do[k], rtol=1e-12, atol=1e-13 * max(scale, 1e-3)), k
+    assert abs(d["kinetic_energy"] - do["kinetic_energy"]) <= 1e-12 * abs(do["kinetic_energy"])
+    H = orc.helicity(w["pos"], w["vort"], w["radius"], wake_n=wn) if okind == orc.HIGHORDER else None
+    if H is not None:
+        assert abs(d["helicity"] - H) <= 1e-10 * max(abs(H), 1e-12)
+        E = orc.enstrophy(w["pos"], w["vort"], w["radius"], wake_n=wn)
+        assert abs(d["enstrophy"] - E) <= 1e-12 * abs(E)
+    else:
+        assert d["enstrophy"] == 0.0
+    print(f"  {d}", flush=True)
+
+
+def test_write_frame_matches_reference_format(tmp_path):
+    a, b, w = _two_wakes()
+    S = pa.Interaction(pa.Wake.from_dict(a), pa.Wake.from_dict(b), nu=1e-3)
+    path = tmp_path / "run.wake"
+    S.write(path, t=0.0, append=False)
+    S.rk4(0.01, 2)
+    S.write(path, t=0.02)
+    st = S.getStates()
+    S.close()
+    raw = path.read_bytes()
+    # parse as scripts/readWake.py does: f64 t, u64 nWake, per wake u64 N, N x 3, N x 3, N
+    off, frames = 0, []
+    while off < len(raw):
+        t = np.frombuffer(raw, np.float64, 1, off)[0]
+        off += 8
+        nw = int(np.frombuffer(raw, np.uint64, 1, off)[0])
+        off += 8
+        wakes = []
+        for _ in range(nw):
+            n = int(np.frombuffer(raw, np.uint64, 1, off)[0])
+            off += 8
+            p = np.frombuffer(raw, np.float64, 3 * n, off).reshape(n, 3)
+            off += 24 * n
+            v = np.frombuffer(raw, np.float64, 3 * n, off).reshape(n, 3)
+            off += 24 * n
+            r = np.frombuffer(raw, np.float64, n, off)
+            off += 8 * n
+            wakes.append((p, v, r))
+        frames.append((t, wakes))
+    assert [f[0] for f in frames] == [0.0, 0.02]
+    (p0, v0, r0), (p1, v1, r1) = frames[0][1]
+    assert np.array_equal(p0, a["pos"]) and np.array_equal(v1, b["vort"])
+    assert np.array_equal(r0, a["radius"]) and np.array_equal(r1, b["radius"])
+    last = frames[1][1]
+    na = a["n"]
+    assert np.array_equal(last[0][0].ravel(), st[:3 * na])
+    assert np.array_equal(last[1][1].ravel(), st[6 * na + 3 * b["n"]:])
+
+
+def test_emission_growing_rotor_wake():
+    """Rotor shedding: every step appends the new row of particles, then one
+    RK4 step; the oracle repeats the same sequence on growing arrays."""
+    nu, dt, steps = 1e-4, 1e-3, 5
+    uinf = np.array([0.0, 0.0, -0.05])
+    e0 = gen.rotor_emission(0)
+    S = pa.Interaction(pa.Wake.from_dict(e0), nu=nu, uinf=tuple(uinf))
+    rad, vol = e0["radius"].copy(), e0["volume"].copy()
+    x = orc.pack_state(e0["pos"], e0["vort"])
+    for k in range(steps):
+        if k > 0:
+            e = gen.rotor_emission(k)
+            S.appendParticles(0, e["pos"], e["vort"], e["radius"], e["volume"])
+            p, v = orc.unpack_state(x)
+            x = orc.pack_state(np.concatenate([p, e["pos"]]), np.concatenate([v, e["vort"]]))
+            rad = np.concatenate([rad, e["radius"]])
+            vol = np.concatenate([vol, e["volume"]])
+        S.rk4(dt, 1)
+        x = orc.rk4(x, rad, vol, nu, dt, 1, mode=orc.OMP_CLEAN, uinf=uinf)
+    g = S.getStates()
+    n = rad.size
+    print(f"case emission: N = {n} after {steps} steps", flush=True)
+    assert S._size == 6 * n
+    ep = per_particle_rel(g[:3 * n], x[:3 * n])
+    ea = per_particle_rel(g[3 * n:], x[3 * n:])
+    print(f"  pos {ep:.3e} vort {ea:.3e}", flush=True)
+    assert ep < 1e-11 and ea < 1e-9
+    # growth past the first allocation (1,280 > 256-particle start) kept the state
+    S.close()
+
+
+def test_emission_into_first_of_two_wakes():
+    a = gen.vortex_ring(16, 16, seed=31)
+    b = gen.vortex_ring(8, 9, seed=32, center=(0, 0, 0.5))
+    e = gen.vortex_ring(4, 16, seed=33, center=(0, 0, -0.3))
+    S = pa.Interaction(pa.Wake.from_dict(a), pa.Wake.from_dict(b), nu=1e-3)
+    S.rk4(0.005, 1)
+    S.appendParticles(0, e["pos"], e["vort"], e["radius"], e["volume"])
+    S.solve()
+    r = S.getRates()
+    st = S.getStates()
+    S.close()
+    n1, n2 = a["n"] + e["n"], b["n"]
+    p, v = orc.unpack_state(st, [n1, n2])
+    rad = np.concatenate([a["radius"], e["radius"], b["radius"]])
+    vol = np.concatenate([a["volume"], e["volume"], b["volume"]])
+    vel, dv = orc.rates(p, v, rad, vol, 1e-3, orc.SERIAL, wake_n=[n1, n2])
+    ref = orc.pack_state(vel, dv, [n1, n2])
+    assert np.max(np.abs(r - ref)) <= 1e-10 * np.max(np.abs(ref))
