@@ -48,9 +48,12 @@ def log(*a):
 STEP_DT = 1e-3  # stable for the tip-vortex swirl (~20 velocity units at 0.016 spacing)
 
 
-def run_path(pa, w, precision, steps, warmup, block=0, chunk=0):
+def run_path(pa, w, precision, steps, warmup, variant=None):
+    v = variant or {}
     S = pa.Interaction(pa.Wake.from_dict(w), nu=w["nu"], precision=precision,
-                       uinf=w.get("uinf", (0, 0, 0)), block=block, chunk=chunk)
+                       uinf=w.get("uinf", (0, 0, 0)), block=v.get("block", 0),
+                       targets_per_thread=v.get("targets_per_thread", 0),
+                       unroll=v.get("unroll", 0))
     dt = STEP_DT
     S.solve()  # builds the device job
     for _ in range(warmup):
@@ -178,6 +181,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner; default variant")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -189,8 +193,16 @@ def main():
 
     w = gen.config_rotor_wake(args.n_shed)
     n = w["n"]
+    tuned = {}
+    if not args.no_tune:
+        for name, prec in (("fp64", pa.FP64), ("fp32c", pa.FP32C)):
+            if name == "fp32c" and args.no_fp32c:
+                continue
+            best, allv = pa.autotune(w, prec, reps=1)
+            tuned[name] = best
+            log(f"autotune {name}: best {best} of {len(allv)} variants")
     log(f"workload rotor wake N={n}; FP64 warmup {args.warmup} steps {args.steps}")
-    S64, r64 = run_path(pa, w, pa.FP64, args.steps, args.warmup)
+    S64, r64 = run_path(pa, w, pa.FP64, args.steps, args.warmup, tuned.get("fp64"))
     log(f"fp64: {r64['pairs_per_s']:.4e} pairs/s, pair kernel {r64['pair_kernel_ms_mean']:.2f} ms")
     achieved = FLOP_PER_PAIR * r64["pairs_per_launch"] / (r64["pair_kernel_ms_mean"] * 1e-3)
     line = {
@@ -214,10 +226,11 @@ def main():
                      "traffic": PAIR64_TRAFFIC_262144 if n == 262144 else None,
                      "kernel": "pair64_kernel", "flop_per_pair": FLOP_PER_PAIR,
                      "launch_ms": r64["pair_kernel_ms_mean"]},
+        "tuned_variants": tuned,
     }
     if not args.no_fp32c:
         log("fp32c path")
-        S32, r32 = run_path(pa, w, pa.FP32C, args.steps, args.warmup)
+        S32, r32 = run_path(pa, w, pa.FP32C, args.steps, args.warmup, tuned.get("fp32c"))
         err = rates_split_error(S64, S32, w)
         line["fp32c"] = {
             "value": r32["pairs_per_s"], "ms_per_step": r32["wall_s"] * 1000.0 / args.steps,

@@ -61,9 +61,18 @@ typedef struct pw_config {
     int precision;    /* PW_FP64 / PW_FP32C                                      */
     double nu;        /* kinematic viscosity (reference: 2.5e-3, interaction.cpp:13) */
     double uinf[3];   /* uniform freestream added to every velocity (0 = none)     */
-    int block;        /* targets per workgroup 64/128/256; 0 = tuned            */
-    int chunk;        /* sources per work item (multiple of block); 0 = tuned   */
+    int block;        /* threads per workgroup 64/128/256; 0 = tuned or default  */
+    int chunk;        /* sources per work item (multiple of block); 0 = policy   */
+    int targets_per_thread; /* 1 or 2; 0 = tuned or default                      */
+    int unroll;       /* source-loop unroll 1/2/4; 0 = tuned or default          */
 } pw_config;
+
+/* A compiled kernel variant of the pair pass. */
+typedef struct pw_variant {
+    int block;              /* 64, 128 or 256 threads per workgroup               */
+    int targets_per_thread; /* 1 or 2                                             */
+    int unroll;             /* 1, 2 or 4; applies to the precision being tuned    */
+} pw_variant;
 
 typedef struct pw_ctx pw_ctx;
 
@@ -170,7 +179,8 @@ typedef struct pw_job_spec {
     int precision;        /* PW_FP64 / PW_FP32C                            */
 } pw_job_spec;
 
-int pw_batch_create(pw_batch **out, int device, int kernel, int block);
+/* variant may be NULL (block 128, one target per thread, default unroll) */
+int pw_batch_create(pw_batch **out, int device, int kernel, const pw_variant *variant);
 int pw_batch_destroy(pw_batch *b);
 const char *pw_batch_last_error(const pw_batch *b);
 /* returns the job id in *id */
@@ -205,7 +215,8 @@ typedef struct pw_server_stats {
     size_t hbm_in_use, hbm_budget;   /* bytes                                 */
 } pw_server_stats;
 
-int pw_server_create(pw_server **out, int device, int kernel, size_t hbm_budget_bytes);
+int pw_server_create(pw_server **out, int device, int kernel, size_t hbm_budget_bytes,
+                     const pw_variant *variant);
 int pw_server_destroy(pw_server *s);
 const char *pw_server_last_error(const pw_server *s);
 /* the server's clock: milliseconds since pw_server_create (steady clock) */
@@ -226,6 +237,26 @@ int pw_server_fetch(pw_server *s, int64_t id, double *pos, double *vort);
 int pw_server_get_stats(pw_server *s, pw_server_stats *st);
 /* device bytes one job of n particles occupies while resident */
 size_t pw_job_footprint(size_t n, int precision);
+
+/* -------------------------------------------------------------- autotuner
+ * Times every compiled variant (block x targets per thread x unroll) of the
+ * pair pass on the given particles (reps solves each, HIP events) and returns
+ * them in `all` (up to max_all entries) and the fastest in *best.  Results
+ * are bitwise identical across variants: only the speed differs. */
+typedef struct pw_tune_entry {
+    pw_variant variant;
+    double pair_ms;      /* mean pair-kernel time per solve       */
+    double pairs_per_s;  /* ordered pair-interactions per second   */
+} pw_tune_entry;
+
+int pw_autotune(int device, int kernel, int precision, size_t n, const double *pos,
+                const double *vort, const double *radius, const double *volume, int reps,
+                pw_tune_entry *all, size_t max_all, size_t *n_all, pw_tune_entry *best);
+/* A tuning cache file: one line per (kernel, precision, log2 N bucket).
+ * pw_create consults the file named by the PW_TUNE_CACHE environment variable
+ * when block, targets_per_thread and unroll are all 0. */
+int pw_tune_save(const char *path, int kernel, int precision, size_t n, const pw_tune_entry *e);
+int pw_tune_lookup(const char *path, int kernel, int precision, size_t n, pw_variant *out);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,7 @@ reference's classes plus the seeded workload generators.
 """
 from . import generators  # noqa: F401
 from ._lib import FP32C, FP64, KERNEL_GAUSSIAN, KERNEL_HIGHORDER, PawanError  # noqa: F401
-from .interaction import RK4, Batch, Integration, Interaction, Server, Wake  # noqa: F401
+from .interaction import RK4, Batch, Integration, Interaction, Server, Wake, autotune  # noqa: F401
 
-__all__ = ["Interaction", "Integration", "RK4", "Batch", "Server", "Wake", "generators",
+__all__ = ["Interaction", "Integration", "RK4", "Batch", "Server", "Wake", "autotune", "generators",
            "FP64", "FP32C", "KERNEL_HIGHORDER", "KERNEL_GAUSSIAN", "PawanError"]

@@ -22,7 +22,15 @@ _dp = C.POINTER(C.c_double)
 class PwConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("kernel", C.c_int), ("precision", C.c_int),
                 ("nu", C.c_double), ("uinf", C.c_double * 3), ("block", C.c_int),
-                ("chunk", C.c_int)]
+                ("chunk", C.c_int), ("targets_per_thread", C.c_int), ("unroll", C.c_int)]
+
+
+class PwVariant(C.Structure):
+    _fields_ = [("block", C.c_int), ("targets_per_thread", C.c_int), ("unroll", C.c_int)]
+
+
+class PwTuneEntry(C.Structure):
+    _fields_ = [("variant", PwVariant), ("pair_ms", C.c_double), ("pairs_per_s", C.c_double)]
 
 
 class PwJobSpec(C.Structure):
@@ -71,7 +79,7 @@ SIGNATURES = {
     "pw_resolve": (C.c_int, [C.c_void_p, _dp]),
     "pw_write": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pw_append_particles": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, _dp, _dp, _dp, _dp]),
-    "pw_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int]),
+    "pw_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(PwVariant)]),
     "pw_batch_destroy": (C.c_int, [C.c_void_p]),
     "pw_batch_last_error": (C.c_char_p, [C.c_void_p]),
     "pw_batch_add": (C.c_int, [C.c_void_p, C.POINTER(PwJobSpec), C.POINTER(C.c_int64)]),
@@ -82,7 +90,8 @@ SIGNATURES = {
     "pw_batch_status": (C.c_int, [C.c_void_p, C.c_int64]),
     "pw_batch_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "pw_chunk_for": (C.c_size_t, [C.c_size_t, C.c_int]),
-    "pw_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_size_t]),
+    "pw_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_size_t,
+                                   C.POINTER(PwVariant)]),
     "pw_server_destroy": (C.c_int, [C.c_void_p]),
     "pw_server_last_error": (C.c_char_p, [C.c_void_p]),
     "pw_server_now_ms": (C.c_double, [C.c_void_p]),
@@ -94,6 +103,11 @@ SIGNATURES = {
     "pw_server_fetch": (C.c_int, [C.c_void_p, C.c_int64, _dp, _dp]),
     "pw_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(PwServerStats)]),
     "pw_job_footprint": (C.c_size_t, [C.c_size_t, C.c_int]),
+    "pw_autotune": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_size_t, _dp, _dp, _dp, _dp, C.c_int,
+                              C.POINTER(PwTuneEntry), C.c_size_t, C.POINTER(C.c_size_t),
+                              C.POINTER(PwTuneEntry)]),
+    "pw_tune_save": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_size_t, C.POINTER(PwTuneEntry)]),
+    "pw_tune_lookup": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_size_t, C.POINTER(PwVariant)]),
 }
 
 _lib = None

@@ -8,6 +8,10 @@
 #include <string>
 #include <vector>
 
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+
 #include "../../include/pawan_hip.h"
 #include "vpm_engine.hpp"
 
@@ -68,13 +72,35 @@ struct pw_ctx {
 
     size_t total() const { return (size_t)woff.back(); }
 
+    pw::Variant choose_variant() const {
+        pw::Variant v;
+        v.block = cfg.block ? cfg.block : default_block(total());
+        v.tpt = cfg.targets_per_thread ? cfg.targets_per_thread : 1;
+        if (cfg.unroll) v.unroll64 = v.unroll32 = cfg.unroll;
+        const char *cache = std::getenv("PW_TUNE_CACHE");
+        if (cache && !cfg.block && !cfg.targets_per_thread && !cfg.unroll) {
+            pw_variant t;
+            if (pw_tune_lookup(cache, cfg.kernel, cfg.precision, total(), &t) == PW_OK) {
+                v.block = t.block;
+                v.tpt = t.targets_per_thread;
+                if (cfg.precision == PW_FP64)
+                    v.unroll64 = t.unroll;
+                else
+                    v.unroll32 = t.unroll;
+            }
+        }
+        return v;
+    }
+
     void rebuild() {
         if (total() == 0) throw Error(PW_EINVAL, "no particles: add a wake first");
-        int tb = cfg.block ? cfg.block : default_block(total());
-        if (!eng || eng->block() != tb) {
+        pw::Variant v = choose_variant();
+        pw::Variant cur = eng ? eng->variant() : pw::Variant{};
+        if (!eng || cur.block != v.block || cur.tpt != v.tpt || cur.unroll64 != v.unroll64 ||
+            cur.unroll32 != v.unroll32) {
             job = nullptr;
             eng.reset();
-            eng.reset(new Engine(cfg.device, cfg.kernel, tb));
+            eng.reset(new Engine(cfg.device, cfg.kernel, v));
         }
         if (job) {
             eng->remove_job(job->id);
@@ -134,6 +160,10 @@ int pw_create(pw_ctx **out, const pw_config *cfg) {
             throw Error(PW_EINVAL, "block must be 0, 64, 128 or 256");
         if (c.chunk < 0 || (c.chunk && c.block && c.chunk % c.block))
             throw Error(PW_EINVAL, "chunk must be a non-negative multiple of block");
+        if (c.targets_per_thread < 0 || c.targets_per_thread > 2)
+            throw Error(PW_EINVAL, "targets_per_thread must be 0, 1 or 2");
+        if (c.unroll != 0 && c.unroll != 1 && c.unroll != 2 && c.unroll != 4)
+            throw Error(PW_EINVAL, "unroll must be 0, 1, 2 or 4");
         if (!finite_params(c.nu, 0.0, c.uinf)) throw Error(PW_EINVAL, "non-finite parameter");
         int ndev = 0;
         pw::hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
@@ -391,12 +421,28 @@ static JobParams params_of(const pw_job_spec *s) {
 
 extern "C" {
 
-int pw_batch_create(pw_batch **out, int device, int kernel, int block) {
+static pw::Variant variant_of(const pw_variant *v, int precision_hint) {
+    pw::Variant r;
+    if (!v) return r;
+    r.block = v->block ? v->block : 128;
+    r.tpt = v->targets_per_thread ? v->targets_per_thread : 1;
+    if (v->unroll) {
+        if (precision_hint == PW_FP32C)
+            r.unroll32 = v->unroll;
+        else if (precision_hint == PW_FP64)
+            r.unroll64 = v->unroll;
+        else
+            r.unroll64 = r.unroll32 = v->unroll;
+    }
+    return r;
+}
+
+int pw_batch_create(pw_batch **out, int device, int kernel, const pw_variant *variant) {
     if (!out) return PW_EINVAL;
     *out = nullptr;
     return guarded(nullptr, [&] {
         auto b = std::make_unique<pw_batch>();
-        b->eng.reset(new Engine(device, kernel, block ? block : 128));
+        b->eng.reset(new Engine(device, kernel, variant_of(variant, -1)));
         *out = b.release();
     });
 }
@@ -521,12 +567,13 @@ extern "C" {
 
 size_t pw_job_footprint(size_t n, int precision) { return pw::job_footprint(n, precision, 128); }
 
-int pw_server_create(pw_server **out, int device, int kernel, size_t hbm_budget_bytes) {
+int pw_server_create(pw_server **out, int device, int kernel, size_t hbm_budget_bytes,
+                     const pw_variant *variant) {
     if (!out) return PW_EINVAL;
     *out = nullptr;
     return guarded(nullptr, [&] {
         auto s = std::make_unique<pw_server>();
-        s->eng.reset(new Engine(device, kernel, 128));
+        s->eng.reset(new Engine(device, kernel, variant_of(variant, -1)));
         s->budget = hbm_budget_bytes;
         *out = s.release();
     });
@@ -683,6 +730,111 @@ int pw_server_get_stats(pw_server *s, pw_server_stats *st) {
     st->hbm_in_use = s->in_use;
     st->hbm_budget = s->budget;
     return PW_OK;
+}
+
+}  // extern "C"
+
+// -------------------------------------------------------------- autotuner
+extern "C" {
+
+int pw_autotune(int device, int kernel, int precision, size_t n, const double *pos,
+                const double *vort, const double *radius, const double *volume, int reps,
+                pw_tune_entry *all, size_t max_all, size_t *n_all, pw_tune_entry *best) {
+    if (!best || !pos || !vort || !radius || !volume || n == 0) return PW_EINVAL;
+    return guarded(nullptr, [&] {
+        if (precision != PW_FP64 && precision != PW_FP32C) throw Error(PW_EINVAL, "precision");
+        const int blocks[3] = {64, 128, 256}, tpts[2] = {1, 2}, unrs[3] = {1, 2, 4};
+        std::vector<pw_tune_entry> res;
+        for (int b : blocks)
+            for (int t : tpts)
+                for (int u : unrs) {
+                    pw::Variant v;
+                    v.block = b;
+                    v.tpt = t;
+                    if (precision == PW_FP64)
+                        v.unroll64 = u;
+                    else
+                        v.unroll32 = u;
+                    if (!pw::variant_exists(kernel, v.block, v.tpt, v.unroll64, v.unroll32)) continue;
+                    Engine eng(device, kernel, v);
+                    JobParams par;
+                    par.precision = precision;
+                    eng.add_job(1, n, pos, vort, radius, volume, par);
+                    eng.solve();  // warm-up (plan, code object load)
+                    double ms = 0.0;
+                    const int r = std::max(1, reps);
+                    for (int k = 0; k < r; ++k) {
+                        eng.solve();
+                        ms += eng.last_pair_ms;
+                    }
+                    pw_tune_entry e;
+                    e.variant.block = b;
+                    e.variant.targets_per_thread = t;
+                    e.variant.unroll = u;
+                    e.pair_ms = ms / r;
+                    e.pairs_per_s = (double)n * (double)(n - 1) / (e.pair_ms * 1e-3);
+                    res.push_back(e);
+                }
+        if (res.empty()) throw Error(PW_EINVAL, "no variant to tune");
+        size_t k = 0;
+        for (size_t i = 1; i < res.size(); ++i)
+            if (res[i].pair_ms < res[k].pair_ms) k = i;
+        *best = res[k];
+        if (all)
+            for (size_t i = 0; i < res.size() && i < max_all; ++i) all[i] = res[i];
+        if (n_all) *n_all = res.size();
+    });
+}
+
+static int n_bucket(size_t n) {
+    int b = 0;
+    while ((size_t)1 << (b + 1) <= n) ++b;
+    return b;
+}
+
+int pw_tune_save(const char *path, int kernel, int precision, size_t n, const pw_tune_entry *e) {
+    if (!path || !e) return PW_EINVAL;
+    return guarded(nullptr, [&] {
+        std::vector<std::string> keep;
+        const int bk = n_bucket(n);
+        {
+            std::ifstream in(path);
+            std::string line;
+            while (std::getline(in, line)) {
+                std::istringstream ss(line);
+                int k, p, b;
+                if (ss >> k >> p >> b && k == kernel && p == precision && b == bk) continue;
+                if (!line.empty()) keep.push_back(line);
+            }
+        }
+        std::ofstream out(path, std::ios::trunc);
+        if (!out) throw Error(PW_EINVAL, "cannot write the tuning cache");
+        for (auto &l : keep) out << l << "\n";
+        out << kernel << " " << precision << " " << bk << " " << e->variant.block << " "
+            << e->variant.targets_per_thread << " " << e->variant.unroll << " " << e->pair_ms << "\n";
+    });
+}
+
+int pw_tune_lookup(const char *path, int kernel, int precision, size_t n, pw_variant *out) {
+    if (!path || !out) return PW_EINVAL;
+    std::ifstream in(path);
+    if (!in) return PW_ENOTFOUND;
+    const int bk = n_bucket(n);
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ss(line);
+        int k, p, b, blk, tpt, unr;
+        if (ss >> k >> p >> b >> blk >> tpt >> unr && k == kernel && p == precision && b == bk) {
+            if (!pw::variant_exists(kernel, blk, tpt, precision == PW_FP64 ? unr : 2,
+                                    precision == PW_FP32C ? unr : 4))
+                return PW_EINVAL;
+            out->block = blk;
+            out->targets_per_thread = tpt;
+            out->unroll = unr;
+            return PW_OK;
+        }
+    }
+    return PW_ENOTFOUND;
 }
 
 }  // extern "C"

@@ -194,14 +194,19 @@ __device__ __forceinline__ int next_item(int *counter, int *slot) {
 
 // ------------------------------------------------------------- pair kernels
 // Persistent: a fixed grid takes (job, target tile, source chunk) items from
-// an atomic queue until it is empty.  Thread = one target; the chunk's
-// sources are staged through LDS, ST records at a time, with coalesced
-// 16-byte loads, and read back as wave-uniform broadcasts.
-template <int KIND, int TB, int ST>
+// an atomic queue until it is empty.  A workgroup of TB threads owns a tile of
+// TB x TPT targets (thread t holds targets t, t + TB, ...); the chunk's
+// sources are staged through LDS, TB records at a time, with coalesced
+// 16-byte loads, and read back as wave-uniform broadcasts, each source serving
+// TPT targets per thread.  UNR is the unroll of the source loop.  Neither TPT
+// nor UNR changes the order of any target's sums.
+template <int KIND, int TB, int TPT, int UNR>
 __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ jobs,
                                                     const Item *__restrict__ items,
                                                     const int *__restrict__ nitems_p,
                                                     int *__restrict__ counter) {
+    constexpr int ST = TB;
+    constexpr int TW = TB * TPT;  // targets per tile
     __shared__ Rec64 tile[ST];
     __shared__ int slot;
     const int nitems = *nitems_p;
@@ -212,15 +217,19 @@ __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ j
         const JobDev &J = jobs[item.job];
         const Rec64 *rec = (const Rec64 *)J.rec;
         const int n = J.n;
-        const int t0 = item.tile * TB;
-        const int i = t0 + (int)threadIdx.x;
-        const bool active = i < n;
-        const Rec64 me = rec[active ? i : n - 1];
+        const int t0 = item.tile * TW;
+        int idx[TPT];
+        Rec64 me[TPT];
+        Acc64 acc[TPT];
+#pragma unroll
+        for (int p = 0; p < TPT; ++p) {
+            idx[p] = t0 + p * TB + (int)threadIdx.x;
+            me[p] = rec[idx[p] < n ? idx[p] : n - 1];
+#pragma unroll
+            for (int m = 0; m < NCOMP; ++m) acc[p].v[m] = 0.0;
+        }
         const double inv_s2 = J.inv_s2;
         const bool uniform = J.uniform != 0;
-        Acc64 acc;
-#pragma unroll
-        for (int m = 0; m < NCOMP; ++m) acc.v[m] = 0.0;
         const int c0 = item.chunk * J.chunk_len;
         const int c1 = min(c0 + J.chunk_len, n);
         for (int s0 = c0; s0 < c1; s0 += ST) {
@@ -232,48 +241,66 @@ __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ j
                 for (int k = threadIdx.x; k < cnt * 4; k += TB) dst[k] = src[k];
             }
             __syncthreads();
-            const bool diag = (s0 < t0 + TB) && (t0 < s0 + cnt);
+            const bool diag = (s0 < t0 + TW) && (t0 < s0 + cnt);
             if (!diag) {
                 if (uniform) {
-#pragma unroll 2
-                    for (int j = 0; j < cnt; ++j)
-                        pair64<KIND, true>(me.x, me.y, me.z, me.ax, me.ay, me.az, me.h, tile[j],
-                                           inv_s2, acc);
+#pragma unroll UNR
+                    for (int j = 0; j < cnt; ++j) {
+                        const Rec64 sj = tile[j];
+#pragma unroll
+                        for (int p = 0; p < TPT; ++p)
+                            pair64<KIND, true>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
+                                               me[p].az, me[p].h, sj, inv_s2, acc[p]);
+                    }
                 } else {
-#pragma unroll 2
-                    for (int j = 0; j < cnt; ++j)
-                        pair64<KIND, false>(me.x, me.y, me.z, me.ax, me.ay, me.az, me.h, tile[j],
-                                            inv_s2, acc);
+#pragma unroll UNR
+                    for (int j = 0; j < cnt; ++j) {
+                        const Rec64 sj = tile[j];
+#pragma unroll
+                        for (int p = 0; p < TPT; ++p)
+                            pair64<KIND, false>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
+                                                me[p].az, me[p].h, sj, inv_s2, acc[p]);
+                    }
                 }
             } else {
-                // the tile holding the block's own targets: skip j == i
+                // a tile holding some of the block's own targets: skip j == i
                 for (int j = 0; j < cnt; ++j) {
-                    if (s0 + j == i) continue;
-                    if (uniform)
-                        pair64<KIND, true>(me.x, me.y, me.z, me.ax, me.ay, me.az, me.h, tile[j],
-                                           inv_s2, acc);
-                    else
-                        pair64<KIND, false>(me.x, me.y, me.z, me.ax, me.ay, me.az, me.h,
-                                            tile[j], inv_s2, acc);
+                    const Rec64 sj = tile[j];
+#pragma unroll
+                    for (int p = 0; p < TPT; ++p) {
+                        if (s0 + j == idx[p]) continue;
+                        if (uniform)
+                            pair64<KIND, true>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
+                                               me[p].az, me[p].h, sj, inv_s2, acc[p]);
+                        else
+                            pair64<KIND, false>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
+                                                me[p].az, me[p].h, sj, inv_s2, acc[p]);
+                    }
                 }
             }
         }
-        if (active) {
-            double *p = J.part + (size_t)item.chunk * NCOMP * J.npad + i;
 #pragma unroll
-            for (int m = 0; m < NCOMP; ++m) p[(size_t)m * J.npad] = acc.v[m];
-        }
+        for (int p = 0; p < TPT; ++p)
+            if (idx[p] < n) {
+                double *o = J.part + (size_t)item.chunk * NCOMP * J.npad + idx[p];
+#pragma unroll
+                for (int m = 0; m < NCOMP; ++m) o[(size_t)m * J.npad] = acc[p].v[m];
+            }
     }
 }
 
 // Compensated binary32 pair pass: plain binary32 sums over SB sources, folded
 // into a (sum, carry) pair per component with TwoSum; the item's result is
-// sum + carry in binary64, and chunks are added in binary64 afterwards.
-template <int KIND, int TB, int ST, int SB>
+// sum + carry in binary64, and chunks are added in binary64 afterwards.  Sub-
+// block boundaries are fixed by the chunk start, whatever TB, TPT and UNR are.
+template <int KIND, int TB, int TPT, int UNR>
 __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ jobs,
                                                     const Item *__restrict__ items,
                                                     const int *__restrict__ nitems_p,
                                                     int *__restrict__ counter) {
+    constexpr int ST = TB;
+    constexpr int SB = 32;
+    constexpr int TW = TB * TPT;
     static_assert(ST % SB == 0, "sub-block must divide the LDS tile");
     __shared__ Rec32 tile[ST];
     __shared__ int slot;
@@ -285,16 +312,21 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
         const JobDev &J = jobs[item.job];
         const Rec32 *rec = (const Rec32 *)J.rec;
         const int n = J.n;
-        const int t0 = item.tile * TB;
-        const int i = t0 + (int)threadIdx.x;
-        const bool active = i < n;
-        const Rec32 me = rec[active ? i : n - 1];
-        const float inv_s2 = (float)J.inv_s2;
-        const float hi = (float)me.h;
-        const bool uniform = J.uniform != 0;
-        Acc32 sum, car;
+        const int t0 = item.tile * TW;
+        int idx[TPT];
+        Rec32 me[TPT];
+        float hi[TPT];
+        Acc32 sum[TPT], car[TPT];
 #pragma unroll
-        for (int m = 0; m < NCOMP; ++m) sum.v[m] = car.v[m] = 0.0f;
+        for (int p = 0; p < TPT; ++p) {
+            idx[p] = t0 + p * TB + (int)threadIdx.x;
+            me[p] = rec[idx[p] < n ? idx[p] : n - 1];
+            hi[p] = (float)me[p].h;
+#pragma unroll
+            for (int m = 0; m < NCOMP; ++m) sum[p].v[m] = car[p].v[m] = 0.0f;
+        }
+        const float inv_s2 = (float)J.inv_s2;
+        const bool uniform = J.uniform != 0;
         const int c0 = item.chunk * J.chunk_len;
         const int c1 = min(c0 + J.chunk_len, n);
         for (int s0 = c0; s0 < c1; s0 += ST) {
@@ -306,45 +338,63 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
                 for (int k = threadIdx.x; k < cnt * 3; k += TB) dst[k] = src[k];
             }
             __syncthreads();
-            const bool diag = (s0 < t0 + TB) && (t0 < s0 + cnt);
+            const bool diag = (s0 < t0 + TW) && (t0 < s0 + cnt);
             for (int b0 = 0; b0 < cnt; b0 += SB) {
                 const int bn = min(SB, cnt - b0);
-                Acc32 a;
+                Acc32 a[TPT];
 #pragma unroll
-                for (int m = 0; m < NCOMP; ++m) a.v[m] = 0.0f;
+                for (int p = 0; p < TPT; ++p)
+#pragma unroll
+                    for (int m = 0; m < NCOMP; ++m) a[p].v[m] = 0.0f;
                 if (!diag && bn == SB) {
                     if (uniform) {
-#pragma unroll 4
-                        for (int j = 0; j < SB; ++j)
-                            pair32<KIND, true>(me.x, me.y, me.z, me.ax, me.ay, me.az, hi,
-                                               tile[b0 + j], inv_s2, a);
+#pragma unroll UNR
+                        for (int j = 0; j < SB; ++j) {
+                            const Rec32 sj = tile[b0 + j];
+#pragma unroll
+                            for (int p = 0; p < TPT; ++p)
+                                pair32<KIND, true>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
+                                                   me[p].az, hi[p], sj, inv_s2, a[p]);
+                        }
                     } else {
-#pragma unroll 2
-                        for (int j = 0; j < SB; ++j)
-                            pair32<KIND, false>(me.x, me.y, me.z, me.ax, me.ay, me.az, hi,
-                                                tile[b0 + j], inv_s2, a);
+#pragma unroll UNR
+                        for (int j = 0; j < SB; ++j) {
+                            const Rec32 sj = tile[b0 + j];
+#pragma unroll
+                            for (int p = 0; p < TPT; ++p)
+                                pair32<KIND, false>(me[p].x, me[p].y, me[p].z, me[p].ax,
+                                                    me[p].ay, me[p].az, hi[p], sj, inv_s2, a[p]);
+                        }
                     }
                 } else {
                     for (int j = 0; j < bn; ++j) {
-                        if (s0 + b0 + j == i) continue;
-                        if (uniform)
-                            pair32<KIND, true>(me.x, me.y, me.z, me.ax, me.ay, me.az, hi,
-                                               tile[b0 + j], inv_s2, a);
-                        else
-                            pair32<KIND, false>(me.x, me.y, me.z, me.ax, me.ay, me.az, hi,
-                                                tile[b0 + j], inv_s2, a);
+                        const Rec32 sj = tile[b0 + j];
+#pragma unroll
+                        for (int p = 0; p < TPT; ++p) {
+                            if (s0 + b0 + j == idx[p]) continue;
+                            if (uniform)
+                                pair32<KIND, true>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
+                                                   me[p].az, hi[p], sj, inv_s2, a[p]);
+                            else
+                                pair32<KIND, false>(me[p].x, me[p].y, me[p].z, me[p].ax,
+                                                    me[p].ay, me[p].az, hi[p], sj, inv_s2, a[p]);
+                        }
                     }
                 }
 #pragma unroll
-                for (int m = 0; m < NCOMP; ++m) two_sum_acc(sum.v[m], car.v[m], a.v[m]);
+                for (int p = 0; p < TPT; ++p)
+#pragma unroll
+                    for (int m = 0; m < NCOMP; ++m) two_sum_acc(sum[p].v[m], car[p].v[m], a[p].v[m]);
             }
         }
-        if (active) {
-            double *p = J.part + (size_t)item.chunk * NCOMP * J.npad + i;
 #pragma unroll
-            for (int m = 0; m < NCOMP; ++m)
-                p[(size_t)m * J.npad] = (double)sum.v[m] + (double)car.v[m];
-        }
+        for (int p = 0; p < TPT; ++p)
+            if (idx[p] < n) {
+                double *o = J.part + (size_t)item.chunk * NCOMP * J.npad + idx[p];
+#pragma unroll
+                for (int m = 0; m < NCOMP; ++m)
+                    o[(size_t)m * J.npad] = (double)sum[p].v[m] + (double)car[p].v[m];
+            }
     }
 }
 

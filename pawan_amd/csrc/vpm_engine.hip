@@ -123,31 +123,54 @@ JobDev Job::dev() const {
 // ------------------------------------------------------------ dispatch
 typedef void (*PairFn)(const JobDev *, const Item *, const int *, int *);
 
-template <int KIND>
-static PairFn pick64(int tb) {
-    switch (tb) {
-    case 64: return pair64_kernel<KIND, 64, 64>;
-    case 128: return pair64_kernel<KIND, 128, 128>;
-    default: return pair64_kernel<KIND, 256, 256>;
+// Variants compiled: every (block, targets per thread, unroll) for the
+// HIGHORDER kernel; the default variant for GAUSSIAN.
+#define PW_VARIANTS(X, KIND)                                                                    \
+    X(KIND, 64, 1, 1) X(KIND, 64, 1, 2) X(KIND, 64, 1, 4) X(KIND, 64, 2, 1) X(KIND, 64, 2, 2)   \
+    X(KIND, 64, 2, 4) X(KIND, 128, 1, 1) X(KIND, 128, 1, 2) X(KIND, 128, 1, 4)                \
+    X(KIND, 128, 2, 1) X(KIND, 128, 2, 2) X(KIND, 128, 2, 4) X(KIND, 256, 1, 1)                \
+    X(KIND, 256, 1, 2) X(KIND, 256, 1, 4) X(KIND, 256, 2, 1) X(KIND, 256, 2, 2)                \
+    X(KIND, 256, 2, 4)
+
+static PairFn pair_fn(int prec, int kind, int tb, int tpt, int unr) {
+#define PW_CASE64(K, B, T, U) \
+    if (tb == B && tpt == T && unr == U) return pair64_kernel<K, B, T, U>;
+#define PW_CASE32(K, B, T, U) \
+    if (tb == B && tpt == T && unr == U) return pair32_kernel<K, B, T, U>;
+    if (kind == KIND_HIGHORDER) {
+        if (prec == PREC_FP64) {
+            PW_VARIANTS(PW_CASE64, KIND_HIGHORDER)
+        } else {
+            PW_VARIANTS(PW_CASE32, KIND_HIGHORDER)
+        }
+    } else {
+        if (prec == PREC_FP64) {
+            PW_CASE64(KIND_GAUSSIAN, 64, 1, 2)
+            PW_CASE64(KIND_GAUSSIAN, 128, 1, 2)
+            PW_CASE64(KIND_GAUSSIAN, 256, 1, 2)
+        } else {
+            PW_CASE32(KIND_GAUSSIAN, 64, 1, 4)
+            PW_CASE32(KIND_GAUSSIAN, 128, 1, 4)
+            PW_CASE32(KIND_GAUSSIAN, 256, 1, 4)
+        }
     }
-}
-template <int KIND>
-static PairFn pick32(int tb) {
-    switch (tb) {
-    case 64: return pair32_kernel<KIND, 64, 64, 32>;
-    case 128: return pair32_kernel<KIND, 128, 128, 32>;
-    default: return pair32_kernel<KIND, 256, 256, 32>;
-    }
-}
-static PairFn pair_fn(int prec, int kind, int tb) {
-    if (prec == PREC_FP64)
-        return kind == KIND_GAUSSIAN ? pick64<KIND_GAUSSIAN>(tb) : pick64<KIND_HIGHORDER>(tb);
-    return kind == KIND_GAUSSIAN ? pick32<KIND_GAUSSIAN>(tb) : pick32<KIND_HIGHORDER>(tb);
+#undef PW_CASE64
+#undef PW_CASE32
+    return nullptr;
 }
 
-Engine::Engine(int device, int kind, int block) : device_(device), kind_(kind), tb_(block) {
+bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32) {
+    return pair_fn(PREC_FP64, kind, tb, tpt, unr64) && pair_fn(PREC_FP32C, kind, tb, tpt, unr32);
+}
+
+Engine::Engine(int device, int kind, const Variant &v)
+    : device_(device), kind_(kind), tb_(v.block), tpt_(v.tpt), unr64_(v.unroll64),
+      unr32_(v.unroll32) {
     if (tb_ != 64 && tb_ != 128 && tb_ != 256) throw Error(PW_EINVAL, "block must be 64, 128 or 256");
     if (kind_ != KIND_HIGHORDER && kind_ != KIND_GAUSSIAN) throw Error(PW_EINVAL, "unknown kernel");
+    fn64_ = pair_fn(PREC_FP64, kind_, tb_, tpt_, unr64_);
+    fn32_ = pair_fn(PREC_FP32C, kind_, tb_, tpt_, unr32_);
+    if (!fn64_ || !fn32_) throw Error(PW_EINVAL, "no compiled kernel variant for these parameters");
     int ndev = 0;
     PW_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) throw Error(PW_EINVAL, "no such HIP device");
@@ -155,10 +178,8 @@ Engine::Engine(int device, int kind, int block) : device_(device), kind_(kind), 
     PW_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     PW_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device_));
     int b64 = 0, b32 = 0;
-    PW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &b64, (const void *)pair_fn(PREC_FP64, kind_, tb_), tb_, 0));
-    PW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &b32, (const void *)pair_fn(PREC_FP32C, kind_, tb_), tb_, 0));
+    PW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b64, (const void *)fn64_, tb_, 0));
+    PW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b32, (const void *)fn32_, tb_, 0));
     grid64_ = std::max(1, b64) * cus_;
     grid32_ = std::max(1, b32) * cus_;
     gridc_ = 4 * cus_;
@@ -206,7 +227,7 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     if (L % (size_t)tb_ != 0) throw Error(PW_EINVAL, "chunk must be a multiple of the block size");
     j->chunk_len = (int)L;
     j->nchunk = (int)((n + L - 1) / L);
-    j->ntile = (int)((n + tb_ - 1) / tb_);
+    j->ntile = (int)((n + tb_ * tpt_ - 1) / (tb_ * tpt_));
     j->chunk_override = chunk_override;
     j->part_chunks = j->nchunk;
     j->uniform = true;
@@ -367,7 +388,7 @@ void Engine::append(Job *j, size_t m, const double *pos, const double *vort, con
     const size_t L = j->chunk_override ? j->chunk_override : chunk_for(n2, tb_);
     j->chunk_len = (int)L;
     j->nchunk = (int)((n2 + L - 1) / L);
-    j->ntile = (int)((n2 + tb_ - 1) / tb_);
+    j->ntile = (int)((n2 + tb_ * tpt_ - 1) / (tb_ * tpt_));
     if (j->nchunk > j->part_chunks) {
         j->part_chunks = std::max(j->nchunk, 16);
         j->part.reset((size_t)j->part_chunks * NCOMP * np * sizeof(double));
@@ -464,7 +485,7 @@ void Engine::launch_stage(int mode, bool timed) {
     auto launch_pair = [&](int prec, int grid, const Item *items, const int *nitems, int *queue,
                            int count) {
         if (!graph && count == 0) return;
-        PairFn fn = pair_fn(prec, kind_, tb_);
+        PairFn fn = prec == PREC_FP64 ? fn64_ : fn32_;
         hipEvent_t a = nullptr, b = nullptr;
         if (timed) {
             size_t k = (size_t)pair_launches * 2;

@@ -84,9 +84,16 @@ struct Job {
     JobDev dev() const;
 };
 
+// Compiled kernel variant: workgroup size, targets per thread, source-loop
+// unroll of the binary64 and binary32 pair kernels.
+struct Variant {
+    int block = 128, tpt = 1, unroll64 = 2, unroll32 = 4;
+};
+bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32);
+
 class Engine {
   public:
-    Engine(int device, int kind, int block);
+    Engine(int device, int kind, const Variant &v);
     ~Engine();
     Engine(const Engine &) = delete;
     Engine &operator=(const Engine &) = delete;
@@ -94,6 +101,7 @@ class Engine {
     int device() const { return device_; }
     int kind() const { return kind_; }
     int block() const { return tb_; }
+    Variant variant() const { return Variant{tb_, tpt_, unr64_, unr32_}; }
     hipStream_t stream() const { return stream_; }
 
     // Create a job from N x 3 positions/vorticities and N radius/volume.
@@ -144,7 +152,9 @@ class Engine {
     void capture_graph();
     double pairs_per_rhs() const;
 
-    int device_, kind_, tb_;
+    int device_, kind_, tb_, tpt_, unr64_, unr32_;
+    void (*fn64_)(const JobDev *, const Item *, const int *, int *) = nullptr;
+    void (*fn32_)(const JobDev *, const Item *, const int *, int *) = nullptr;
     hipStream_t stream_ = nullptr;
     int cus_ = 0;
     int grid64_ = 0, grid32_ = 0, gridc_ = 0;

@@ -70,13 +70,14 @@ class Interaction:
 
     def __init__(self, *wakes, nu: float = 2.5e-3, kernel: int = _lib.KERNEL_HIGHORDER,
                  precision: int = _lib.FP64, uinf=(0.0, 0.0, 0.0), device: int = 0,
-                 block: int = 0, chunk: int = 0):
+                 block: int = 0, chunk: int = 0, targets_per_thread: int = 0, unroll: int = 0):
         L = lib()
         cfg = _lib.PwConfig()
         L.pw_config_default(C.byref(cfg))
         cfg.device, cfg.kernel, cfg.precision, cfg.nu = device, kernel, precision, nu
         cfg.uinf[:] = list(uinf)
         cfg.block, cfg.chunk = block, chunk
+        cfg.targets_per_thread, cfg.unroll = targets_per_thread, unroll
         h = C.c_void_p()
         rc = L.pw_create(C.byref(h), C.byref(cfg))
         if rc != _lib.PW_OK:
@@ -261,9 +262,11 @@ class RK4(Integration):
 class Batch:
     """Many independent simulations stepped in one persistent launch."""
 
-    def __init__(self, device: int = 0, kernel: int = _lib.KERNEL_HIGHORDER, block: int = 0):
+    def __init__(self, device: int = 0, kernel: int = _lib.KERNEL_HIGHORDER, block: int = 0,
+                 targets_per_thread: int = 0, unroll: int = 0):
         h = C.c_void_p()
-        rc = lib().pw_batch_create(C.byref(h), device, kernel, block)
+        v = _lib.PwVariant(block, targets_per_thread, unroll)
+        rc = lib().pw_batch_create(C.byref(h), device, kernel, C.byref(v))
         if rc != _lib.PW_OK:
             raise _lib.PawanError(rc, lib().pw_batch_last_error(None).decode())
         self._h = h
@@ -334,9 +337,11 @@ class Server:
     QUEUED, RUNNING, DONE, FAILED, CANCELLED = 0, 1, 2, 3, 4
 
     def __init__(self, hbm_budget_bytes: int, device: int = 0,
-                 kernel: int = _lib.KERNEL_HIGHORDER):
+                 kernel: int = _lib.KERNEL_HIGHORDER, block: int = 0, targets_per_thread: int = 0,
+                 unroll: int = 0):
         h = C.c_void_p()
-        rc = lib().pw_server_create(C.byref(h), device, kernel, hbm_budget_bytes)
+        v = _lib.PwVariant(block, targets_per_thread, unroll)
+        rc = lib().pw_server_create(C.byref(h), device, kernel, hbm_budget_bytes, C.byref(v))
         if rc != _lib.PW_OK:
             raise _lib.PawanError(rc, lib().pw_server_last_error(None).decode())
         self._h = h
@@ -390,3 +395,27 @@ class Server:
         st = _lib.PwServerStats()
         self._ck(lib().pw_server_get_stats(self._h, C.byref(st)))
         return {f: getattr(st, f) for f, _ in st._fields_}
+
+
+def autotune(w: dict, precision: int = _lib.FP64, kernel: int = _lib.KERNEL_HIGHORDER,
+             reps: int = 2, device: int = 0, cache: str = None):
+    """Time every compiled pair-kernel variant on the particles of ``w`` and
+    return (best, all) as dicts; optionally record the best in a tuning cache
+    file (read by Interaction through the PW_TUNE_CACHE variable)."""
+    keep = [_f64(w[k]) for k in ("pos", "vort", "radius", "volume")]
+    n = keep[0].reshape(-1, 3).shape[0]
+    allv = (_lib.PwTuneEntry * 64)()
+    cnt = C.c_size_t()
+    best = _lib.PwTuneEntry()
+    rc = lib().pw_autotune(device, kernel, precision, n, *[_lib.ptr(a) for a in keep], reps, allv,
+                           64, C.byref(cnt), C.byref(best))
+    if rc != _lib.PW_OK:
+        raise _lib.PawanError(rc, lib().pw_last_error(None).decode())
+
+    def d(e):
+        return {"block": e.variant.block, "targets_per_thread": e.variant.targets_per_thread,
+                "unroll": e.variant.unroll, "pair_ms": e.pair_ms, "pairs_per_s": e.pairs_per_s}
+
+    if cache:
+        lib().pw_tune_save(str(cache).encode(), kernel, precision, n, C.byref(best))
+    return d(best), [d(allv[i]) for i in range(cnt.value)]

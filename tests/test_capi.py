@@ -81,3 +81,22 @@ def test_no_device_is_an_error_not_a_crash():
     rc = lib.pw_create(C.byref(h), None)
     assert rc in (_lib.PW_EHIP, _lib.PW_EINVAL) and h.value is None
     assert lib.pw_last_error(None)
+
+
+def test_tune_cache_roundtrip(tmp_path):
+    lib = _lib.load()
+    path = str(tmp_path / "tune.txt").encode()
+    v = _lib.PwVariant()
+    assert lib.pw_tune_lookup(path, 0, 0, 262144, C.byref(v)) == _lib.PW_ENOTFOUND
+    e = _lib.PwTuneEntry(_lib.PwVariant(256, 2, 4), 123.0, 5e11)
+    assert lib.pw_tune_save(path, 0, 0, 262144, C.byref(e)) == 0
+    e2 = _lib.PwTuneEntry(_lib.PwVariant(64, 1, 1), 1.0, 1e9)
+    assert lib.pw_tune_save(path, 0, 1, 2048, C.byref(e2)) == 0
+    e3 = _lib.PwTuneEntry(_lib.PwVariant(128, 1, 2), 120.0, 5e11)
+    assert lib.pw_tune_save(path, 0, 0, 300000, C.byref(e3)) == 0  # same log2 bucket: replaces
+    assert lib.pw_tune_lookup(path, 0, 0, 262144, C.byref(v)) == 0
+    assert (v.block, v.targets_per_thread, v.unroll) == (128, 1, 2)
+    assert lib.pw_tune_lookup(path, 0, 1, 3000, C.byref(v)) == 0
+    assert (v.block, v.targets_per_thread, v.unroll) == (64, 1, 1)
+    assert lib.pw_tune_lookup(path, 1, 0, 262144, C.byref(v)) == _lib.PW_ENOTFOUND
+    assert len(open(path.decode()).read().splitlines()) == 2

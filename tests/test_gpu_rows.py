@@ -42,9 +42,13 @@ def test_diagnose_all(kernel, okind):
     S.close()
     wn = [a["n"], b["n"]]
     do = orc.diagnostics(w["pos"], w["vort"], w["radius"], wake_n=wn)
-    for k in ("total_vorticity", "linear_impulse", "angular_impulse"):
-        scale = np.abs(do[k]).max() + 1e-300
-        assert np.allclose(d[k], do[k], rtol=1e-12, atol=1e-13 * max(scale, 1e-3)), k
+    # the sums cancel (a ring's total vorticity is ~0), so compare against the
+    # scale of the summands: sum |a|, sum |x||a|, sum |x|^2 |a|
+    na = np.linalg.norm(w["vort"], axis=1)
+    nx = np.linalg.norm(w["pos"], axis=1)
+    for k, scale in (("total_vorticity", na.sum()), ("linear_impulse", (nx * na).sum()),
+                     ("angular_impulse", (nx * nx * na).sum())):
+        assert np.abs(d[k] - do[k]).max() <= 1e-14 * scale, (k, d[k], do[k])
     assert abs(d["kinetic_energy"] - do["kinetic_energy"]) <= 1e-12 * abs(do["kinetic_energy"])
     H = orc.helicity(w["pos"], w["vort"], w["radius"], wake_n=wn) if okind == orc.HIGHORDER else None
     if H is not None:
