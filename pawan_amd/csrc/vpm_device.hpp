@@ -45,11 +45,12 @@ __device__ __forceinline__ void kernel64(double t, double rt, double &Q, double 
     if (KIND == KIND_HIGHORDER) {
         const double w = rsq_nr(t + 1.0);  // (t+1)^-1/2
         const double iu = w * w;
-        const double iu2 = iu * iu;
-        const double iu4 = iu2 * iu2;
-        Zs = K_CZ * (iu4 * iu2 * iu * w);  // (t+1)^-7.5
-        const double phir = __builtin_fma(t, K_CPHI, 1.5 * K_CPHI) * (iu2 * rt);
-        Q = (phir - Zs) * it;
+        const double iu2 = iu * iu;        // (t+1)^-2
+        const double w5 = iu2 * w;
+        Zs = K_CZ * (w5 * w5 * w5);        // (1.875/pi) (t+1)^-7.5
+        // phi/rb - Z = (0.25/pi)(t+1.5)(t+1)^-2 t^-1/2 - Z
+        const double P = __builtin_fma(__builtin_fma(t, K_CPHI, 1.5 * K_CPHI), iu2 * rt, -Zs);
+        Q = P * it;
     } else {
         Zs = K_CG * exp(-0.5 * t);
         const double phir = K_CPHI * erf(K_SQRT1_2 * (t * rt)) * rt;
@@ -65,10 +66,11 @@ __device__ __forceinline__ void kernel32(float t, float rt, float &Q, float &Fp,
         const float w = __builtin_amdgcn_rsqf(t + 1.0f);
         const float iu = w * w;
         const float iu2 = iu * iu;
-        const float iu4 = iu2 * iu2;
-        Zs = (float)K_CZ * (iu4 * iu2 * iu * w);
-        const float phir = __builtin_fmaf(t, (float)K_CPHI, (float)(1.5 * K_CPHI)) * (iu2 * rt);
-        Q = (phir - Zs) * it;
+        const float w5 = iu2 * w;
+        Zs = (float)K_CZ * (w5 * w5 * w5);
+        const float P =
+            __builtin_fmaf(__builtin_fmaf(t, (float)K_CPHI, (float)(1.5 * K_CPHI)), iu2 * rt, -Zs);
+        Q = P * it;
     } else {
         Zs = (float)K_CG * __expf(-0.5f * t);
         const float phir = (float)K_CPHI * erff((float)K_SQRT1_2 * (t * rt)) * rt;
@@ -82,7 +84,8 @@ struct Acc64 {
 };
 
 // One pair in binary64.  UNIFORM: every particle of the job has the same
-// core radius, so sigma^-3, sigma^-5 are applied once in the epilogue.
+// core radius; positions are then stored divided by sigma, so t = |d|^2 and
+// the powers of sigma are applied once in the epilogue.
 template <int KIND, bool UNIFORM>
 __device__ __forceinline__ void pair64(double xi, double yi, double zi, double aix, double aiy,
                                        double aiz, double hi, const Rec64 &s, double inv_s2,
@@ -91,8 +94,7 @@ __device__ __forceinline__ void pair64(double xi, double yi, double zi, double a
     const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
     double Q, Fp, Zs;
     if (UNIFORM) {
-        const double t = r2 * inv_s2;
-        kernel64<KIND>(t, rsq_nr(t), Q, Fp, Zs);
+        kernel64<KIND>(r2, rsq_nr(r2), Q, Fp, Zs);
     } else {
         const double isg = rsq_nr(hi + s.h);  // 1/sigma
         const double is2 = isg * isg;
@@ -131,16 +133,19 @@ struct Acc32 {
     float v[NCOMP];
 };
 
+// One pair in binary32.  Formed in binary64 and rounded: the displacement,
+// and the stretching cross product a_i x a_j (nearly parallel strengths along
+// a filament cancel in it; from binary32 strengths the error would be
+// eps32 |a|^2 per pair, which the cancelling sums over sources then expose).
 template <int KIND, bool UNIFORM>
-__device__ __forceinline__ void pair32(double xi, double yi, double zi, float aix, float aiy,
-                                       float aiz, float hi, const Rec32 &s, float inv_s2,
+__device__ __forceinline__ void pair32(double xi, double yi, double zi, double aix, double aiy,
+                                       double aiz, float hi, const Rec32 &s, float inv_s2,
                                        Acc32 &a) {
     const float dx = (float)(xi - s.x), dy = (float)(yi - s.y), dz = (float)(zi - s.z);
     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
     float Q, Fp, Zs;
     if (UNIFORM) {
-        const float t = r2 * inv_s2;
-        kernel32<KIND>(t, __builtin_amdgcn_rsqf(t), Q, Fp, Zs);
+        kernel32<KIND>(r2, __builtin_amdgcn_rsqf(r2), Q, Fp, Zs);
     } else {
         const float isg = __builtin_amdgcn_rsqf(hi + (float)s.h);
         const float is2 = isg * isg;
@@ -151,15 +156,15 @@ __device__ __forceinline__ void pair32(double xi, double yi, double zi, float ai
         Fp *= s5;
         Zs *= s5;
     }
-    const float cx = __builtin_fmaf(s.ay, dz, -s.az * dy);
-    const float cy = __builtin_fmaf(s.az, dx, -s.ax * dz);
-    const float cz = __builtin_fmaf(s.ax, dy, -s.ay * dx);
+    const float cx = __builtin_fmaf(s.fy, dz, -s.fz * dy);
+    const float cy = __builtin_fmaf(s.fz, dx, -s.fx * dz);
+    const float cz = __builtin_fmaf(s.fx, dy, -s.fy * dx);
     a.v[0] = __builtin_fmaf(Q, cx, a.v[0]);
     a.v[1] = __builtin_fmaf(Q, cy, a.v[1]);
     a.v[2] = __builtin_fmaf(Q, cz, a.v[2]);
-    const float ex = __builtin_fmaf(aiy, s.az, -aiz * s.ay);
-    const float ey = __builtin_fmaf(aiz, s.ax, -aix * s.az);
-    const float ez = __builtin_fmaf(aix, s.ay, -aiy * s.ax);
+    const float ex = (float)__builtin_fma(aiy, s.az, -aiz * s.ay);
+    const float ey = (float)__builtin_fma(aiz, s.ax, -aix * s.az);
+    const float ez = (float)__builtin_fma(aix, s.ay, -aiy * s.ax);
     a.v[3] = __builtin_fmaf(Q, ex, a.v[3]);
     a.v[4] = __builtin_fmaf(Q, ey, a.v[4]);
     a.v[5] = __builtin_fmaf(Q, ez, a.v[5]);
@@ -167,9 +172,9 @@ __device__ __forceinline__ void pair32(double xi, double yi, double zi, float ai
     a.v[6] = __builtin_fmaf(fd, dx, a.v[6]);
     a.v[7] = __builtin_fmaf(fd, dy, a.v[7]);
     a.v[8] = __builtin_fmaf(fd, dz, a.v[8]);
-    a.v[9] = __builtin_fmaf(Zs, s.ax, a.v[9]);
-    a.v[10] = __builtin_fmaf(Zs, s.ay, a.v[10]);
-    a.v[11] = __builtin_fmaf(Zs, s.az, a.v[11]);
+    a.v[9] = __builtin_fmaf(Zs, s.fx, a.v[9]);
+    a.v[10] = __builtin_fmaf(Zs, s.fy, a.v[10]);
+    a.v[11] = __builtin_fmaf(Zs, s.fz, a.v[11]);
     a.v[12] = __builtin_fmaf(Zs, s.v, a.v[12]);
 }
 
@@ -335,7 +340,7 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
             {
                 const float4 *src = (const float4 *)(rec + s0);
                 float4 *dst = (float4 *)tile;
-                for (int k = threadIdx.x; k < cnt * 3; k += TB) dst[k] = src[k];
+                for (int k = threadIdx.x; k < cnt * 5; k += TB) dst[k] = src[k];
             }
             __syncthreads();
             const bool diag = (s0 < t0 + TW) && (t0 < s0 + cnt);
@@ -403,30 +408,33 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
 // rates
 //   u   = su * S[0:3] + uinf
 //   da  = sq * S[3:6] + sf * S[6:9] + sd * (v_i S[9:12] - a_i S[12])
-// (uniform core: su = sq = sigma^-3, sf = sigma^-5, sd = 2 nu sigma^-5;
-// otherwise the per-pair factors are already inside and su = sq = sf = 1,
-// sd = 2 nu), then apply the RK4 / Euler stage combination of rk4.cpp:21-66
+// (uniform core, positions prescaled by 1/sigma: su = sigma^-2,
+// sq = sf = sigma^-3, sd = 2 nu sigma^-5; otherwise the per-pair factors are
+// already inside and su = sq = sf = 1, sd = 2 nu), then apply the RK4 / Euler stage combination of rk4.cpp:21-66
 // with unfused multiplies and adds in the reference's order, and write the
 // next stage's record.  Block 0 also re-arms the pair kernels' queues.
 template <typename Rec>
-__device__ __forceinline__ void store_rec(Rec *r, int i, const double *xs);
+__device__ __forceinline__ void store_rec(Rec *r, int i, const double *xs, double scale);
 template <>
-__device__ __forceinline__ void store_rec<Rec64>(Rec64 *r, int i, const double *xs) {
-    r[i].x = xs[0];
-    r[i].y = xs[1];
-    r[i].z = xs[2];
+__device__ __forceinline__ void store_rec<Rec64>(Rec64 *r, int i, const double *xs, double scale) {
+    r[i].x = xs[0] * scale;
+    r[i].y = xs[1] * scale;
+    r[i].z = xs[2] * scale;
     r[i].ax = xs[3];
     r[i].ay = xs[4];
     r[i].az = xs[5];
 }
 template <>
-__device__ __forceinline__ void store_rec<Rec32>(Rec32 *r, int i, const double *xs) {
-    r[i].x = xs[0];
-    r[i].y = xs[1];
-    r[i].z = xs[2];
-    r[i].ax = (float)xs[3];
-    r[i].ay = (float)xs[4];
-    r[i].az = (float)xs[5];
+__device__ __forceinline__ void store_rec<Rec32>(Rec32 *r, int i, const double *xs, double scale) {
+    r[i].x = xs[0] * scale;
+    r[i].y = xs[1] * scale;
+    r[i].z = xs[2] * scale;
+    r[i].ax = xs[3];
+    r[i].ay = xs[4];
+    r[i].az = xs[5];
+    r[i].fx = (float)xs[3];
+    r[i].fy = (float)xs[4];
+    r[i].fz = (float)xs[5];
 }
 
 __global__ __launch_bounds__(256) void combine_kernel(const JobDev *__restrict__ jobs,
@@ -502,9 +510,9 @@ __global__ __launch_bounds__(256) void combine_kernel(const JobDev *__restrict__
         }
         if (!finite) atomicOr(J.flag, 1);
         if (J.prec == PREC_FP64)
-            store_rec<Rec64>((Rec64 *)J.rec, i, nx);
+            store_rec<Rec64>((Rec64 *)J.rec, i, nx, J.rec_scale);
         else
-            store_rec<Rec32>((Rec32 *)J.rec, i, nx);
+            store_rec<Rec32>((Rec32 *)J.rec, i, nx, J.rec_scale);
     }
 }
 
@@ -516,9 +524,9 @@ __global__ __launch_bounds__(256) void refresh_rec_kernel(JobDev J) {
     double xs[6];
     for (int m = 0; m < 6; ++m) xs[m] = J.xs[m * np + i];
     if (J.prec == PREC_FP64)
-        store_rec<Rec64>((Rec64 *)J.rec, i, xs);
+        store_rec<Rec64>((Rec64 *)J.rec, i, xs, J.rec_scale);
     else
-        store_rec<Rec32>((Rec32 *)J.rec, i, xs);
+        store_rec<Rec32>((Rec32 *)J.rec, i, xs, J.rec_scale);
 }
 
 // Copy x0 -> xs (a new step or a new state starts from x0).

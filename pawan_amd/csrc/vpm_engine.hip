@@ -70,6 +70,37 @@ static size_t rec_bytes(int precision) {
     return precision == PREC_FP64 ? sizeof(Rec64) : sizeof(Rec32);
 }
 
+static void fill_records(void *dst, int precision, size_t m, const double *pos, const double *vort,
+                         const double *radius, const double *volume, double scale) {
+    for (size_t i = 0; i < m; ++i) {
+        if (precision == PREC_FP64) {
+            Rec64 &r = ((Rec64 *)dst)[i];
+            r.x = pos[3 * i] * scale;
+            r.y = pos[3 * i + 1] * scale;
+            r.z = pos[3 * i + 2] * scale;
+            r.h = 0.5 * radius[i] * radius[i];
+            r.ax = vort[3 * i];
+            r.ay = vort[3 * i + 1];
+            r.az = vort[3 * i + 2];
+            r.v = volume[i];
+        } else {
+            Rec32 &r = ((Rec32 *)dst)[i];
+            r.x = pos[3 * i] * scale;
+            r.y = pos[3 * i + 1] * scale;
+            r.z = pos[3 * i + 2] * scale;
+            r.h = 0.5 * radius[i] * radius[i];
+            r.ax = vort[3 * i];
+            r.ay = vort[3 * i + 1];
+            r.az = vort[3 * i + 2];
+            r.fx = (float)vort[3 * i];
+            r.fy = (float)vort[3 * i + 1];
+            r.fz = (float)vort[3 * i + 2];
+            r.v = (float)volume[i];
+            r.pad[0] = r.pad[1] = 0.0f;
+        }
+    }
+}
+
 size_t job_footprint(size_t n, int precision, int TB) {
     size_t npad = round_up(std::max<size_t>(n, 1), 256);
     size_t L = chunk_for(n, TB);
@@ -102,15 +133,17 @@ JobDev Job::dev() const {
     d.dt = par.dt;
     d.nu = par.nu;
     for (int c = 0; c < 3; ++c) d.uinf[c] = par.uinf[c];
+    d.rec_scale = rec_scale();
     if (uniform) {
-        // sigma^2 = (s^2 + s^2)/2 = s^2
+        // sigma^2 = (s^2 + s^2)/2 = s^2; records hold x / sigma, so
+        // d = sigma d^, q = sigma^-3 Q, F = sigma^-5 F', Z = sigma^-3 Zs
         long double s2 = (long double)sigma * sigma;
         long double sg = sqrtl(s2);
         long double s3 = s2 * sg, s5 = s3 * s2;
-        d.inv_s2 = (double)(1.0L / s2);
-        d.su = (double)(1.0L / s3);
-        d.sq = d.su;
-        d.sf = (double)(1.0L / s5);
+        d.inv_s2 = 1.0;
+        d.su = (double)(1.0L / s2);
+        d.sq = (double)(1.0L / s3);
+        d.sf = d.sq;
         d.sd = (double)(2.0L * par.nu / s5);
     } else {
         d.inv_s2 = 0.0;
@@ -247,33 +280,8 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     // host-side staging of records, state and constants
     size_t need = std::max(np * rec_bytes(par.precision), 6 * np * sizeof(double));
     stage_.ensure(std::max(need, 2 * np * sizeof(double)));
-    if (par.precision == PREC_FP64) {
-        Rec64 *r = stage_.as<Rec64>();
-        std::memset(r, 0, np * sizeof(Rec64));
-        for (size_t i = 0; i < n; ++i) {
-            r[i].x = pos[3 * i];
-            r[i].y = pos[3 * i + 1];
-            r[i].z = pos[3 * i + 2];
-            r[i].h = 0.5 * radius[i] * radius[i];
-            r[i].ax = vort[3 * i];
-            r[i].ay = vort[3 * i + 1];
-            r[i].az = vort[3 * i + 2];
-            r[i].v = volume[i];
-        }
-    } else {
-        Rec32 *r = stage_.as<Rec32>();
-        std::memset(r, 0, np * sizeof(Rec32));
-        for (size_t i = 0; i < n; ++i) {
-            r[i].x = pos[3 * i];
-            r[i].y = pos[3 * i + 1];
-            r[i].z = pos[3 * i + 2];
-            r[i].h = 0.5 * radius[i] * radius[i];
-            r[i].ax = (float)vort[3 * i];
-            r[i].ay = (float)vort[3 * i + 1];
-            r[i].az = (float)vort[3 * i + 2];
-            r[i].v = (float)volume[i];
-        }
-    }
+    std::memset(stage_.p, 0, np * rec_bytes(par.precision));
+    fill_records(stage_.p, par.precision, n, pos, vort, radius, volume, j->rec_scale());
     PW_HIP(hipMemcpyAsync(j->rec.p, stage_.p, np * rec_bytes(par.precision), hipMemcpyHostToDevice,
                           stream_));
     PW_HIP(hipStreamSynchronize(stream_));
@@ -292,33 +300,6 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     plan_dirty_ = true;
     upload_state_soa(raw, pos, vort);
     return raw;
-}
-
-static void fill_records(void *dst, int precision, size_t m, const double *pos, const double *vort,
-                         const double *radius, const double *volume) {
-    for (size_t i = 0; i < m; ++i) {
-        if (precision == PREC_FP64) {
-            Rec64 &r = ((Rec64 *)dst)[i];
-            r.x = pos[3 * i];
-            r.y = pos[3 * i + 1];
-            r.z = pos[3 * i + 2];
-            r.h = 0.5 * radius[i] * radius[i];
-            r.ax = vort[3 * i];
-            r.ay = vort[3 * i + 1];
-            r.az = vort[3 * i + 2];
-            r.v = volume[i];
-        } else {
-            Rec32 &r = ((Rec32 *)dst)[i];
-            r.x = pos[3 * i];
-            r.y = pos[3 * i + 1];
-            r.z = pos[3 * i + 2];
-            r.h = 0.5 * radius[i] * radius[i];
-            r.ax = (float)vort[3 * i];
-            r.ay = (float)vort[3 * i + 1];
-            r.az = (float)vort[3 * i + 2];
-            r.v = (float)volume[i];
-        }
-    }
 }
 
 void Engine::append(Job *j, size_t m, const double *pos, const double *vort, const double *radius,
@@ -356,7 +337,17 @@ void Engine::append(Job *j, size_t m, const double *pos, const double *vort, con
     const size_t np = j->npad;
     // new particles' records, state (x0 = xs between steps), volume, radius
     stage_.ensure(std::max(m * rb, 8 * m * sizeof(double)));
-    fill_records(stage_.p, j->par.precision, m, pos, vort, radius, volume);
+    bool uni = j->uniform;
+    for (size_t i = 0; i < m && uni; ++i)
+        if (std::memcmp(&radius[i], &j->sigma, sizeof(double)) != 0) uni = false;
+    if (uni != j->uniform) {  // the job loses its uniform core: records back to plain positions
+        j->uniform = false;
+        hipLaunchKernelGGL(refresh_rec_kernel, dim3((n + 255) / 256), dim3(256), 0, stream_,
+                           j->dev());
+        PW_HIP(hipGetLastError());
+        PW_HIP(hipStreamSynchronize(stream_));
+    }
+    fill_records(stage_.p, j->par.precision, m, pos, vort, radius, volume, j->rec_scale());
     PW_HIP(hipMemcpyAsync(j->rec.as<char>() + n * rb, stage_.p, m * rb, hipMemcpyHostToDevice,
                           stream_));
     PW_HIP(hipStreamSynchronize(stream_));
@@ -382,8 +373,6 @@ void Engine::append(Job *j, size_t m, const double *pos, const double *vort, con
     PW_HIP(hipMemcpyAsync(j->vol.as<double>() + np + n, h + 7 * m, m * sizeof(double),
                           hipMemcpyHostToDevice, stream_));
     PW_HIP(hipStreamSynchronize(stream_));
-    for (size_t i = 0; i < m; ++i)
-        if (std::memcmp(&radius[i], &j->sigma, sizeof(double)) != 0) j->uniform = false;
     j->n = n2;
     const size_t L = j->chunk_override ? j->chunk_override : chunk_for(n2, tb_);
     j->chunk_len = (int)L;

@@ -82,6 +82,8 @@ struct Job {
     size_t chunk_override = 0;  // fixed chunk length (0: policy)
     int part_chunks = 0;        // chunks the partial-sum buffer holds
     JobDev dev() const;
+    // factor applied to positions in the records (1/sigma for a uniform core)
+    double rec_scale() const { return uniform ? 1.0 / sigma : 1.0; }
 };
 
 // Compiled kernel variant: workgroup size, targets per thread, source-loop

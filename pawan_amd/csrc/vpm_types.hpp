@@ -36,11 +36,19 @@ struct __align__(16) Rec64 {
     double ax, ay, az, v;
 };
 // Record of the compensated binary32 path: binary64 positions (the
-// displacement is formed in binary64 and then rounded), binary32 strengths.
+// displacement is formed in binary64 and then rounded) and binary64
+// strengths (the stretching cross product a_i x a_j is formed in binary64:
+// nearly parallel strengths would lose it in binary32), plus a binary32 copy
+// of the strengths for the velocity and diffusion terms.  80 bytes, 5 x 16 B.
 struct __align__(16) Rec32 {
     double x, y, z, h;
-    float ax, ay, az, v;
+    double ax, ay, az;
+    float fx, fy, fz, v;
+    float pad[2];
 };
+
+static_assert(sizeof(Rec64) == 64, "Rec64 is four 16-byte vectors");
+static_assert(sizeof(Rec32) == 80, "Rec32 is five 16-byte vectors");
 
 // One simulation as the kernels see it.  Arrays are structure-of-arrays with
 // a padded stride npad (multiple of 256): component m of particle i is at
@@ -57,11 +65,11 @@ struct JobDev {
     int *flag;      // set to 1 when a rate or state is non-finite
     int n, npad, nchunk, chunk_len;
     int prec, uniform, pad0, pad1;
-    double inv_s2;  // uniform core: 1/sigma^2
+    double inv_s2;  // non-uniform core: unused; uniform: 1 (positions are prescaled)
+    double rec_scale;  // records hold positions x rec_scale (1/sigma for a uniform core)
     double su, sq, sf, sd;  // epilogue scales (see combine_kernel)
     double dt, nu;
     double uinf[3];
-    double pad2;
 };
 
 struct Item {
