@@ -125,10 +125,12 @@ def cpu_baseline(w, budget_s=15.0):
     }
 
 
-def serving_run(pa, gen, duration_s=20.0, rate_hz=4.0, seed=1, budget=200 << 30):
+def serving_run(pa, gen, duration_s=20.0, rate_hz=4.0, seed=1, budget=200 << 30, n_lo=2048,
+                n_hi=32768, s_lo=50, s_hi=200, drain_factor=3.0):
     """Poisson arrivals for duration_s; ring or rotor-wake jobs with N
-    log-uniform in [2048, 32768], steps uniform in [50, 200], precision
-    50/50; runs until all arrived jobs finish or 3 x duration_s passes."""
+    log-uniform in [n_lo, n_hi], steps uniform in [s_lo, s_hi], precision
+    50/50; runs until all arrived jobs finish or drain_factor x duration_s
+    passes (jobs still running then are not counted as finished)."""
     u = gen.uniform(seed, 4096)
     srv = pa.Server(budget)
     t_next, k, ids = 0.0, 0, []
@@ -137,8 +139,8 @@ def serving_run(pa, gen, duration_s=20.0, rate_hz=4.0, seed=1, budget=200 << 30)
         now = srv.now_ms()
         while now >= t_next and t_next < t_end:
             a = u[(6 * k) % 4090: (6 * k) % 4090 + 6]
-            n = int(2048 * (32768 / 2048) ** a[0])
-            steps = int(50 + 150 * a[1])
+            n = int(n_lo * (n_hi / n_lo) ** a[0])
+            steps = int(s_lo + (s_hi - s_lo) * a[1])
             prec = pa.FP64 if a[2] < 0.5 else pa.FP32C
             if a[3] < 0.5:
                 n_cs = gen.ring_cs_for(n)
@@ -154,7 +156,7 @@ def serving_run(pa, gen, duration_s=20.0, rate_hz=4.0, seed=1, budget=200 << 30)
         st = srv.stats()
         if t_next >= t_end and running == 0 and st["queued"] == 0:
             break
-        if srv.now_ms() > 3 * t_end:
+        if srv.now_ms() > drain_factor * t_end:
             break
         if running == 0 and st["queued"] == 0:
             time.sleep(max(0.0, min(t_next - srv.now_ms(), 50.0)) / 1000.0)
@@ -166,8 +168,170 @@ def serving_run(pa, gen, duration_s=20.0, rate_hz=4.0, seed=1, budget=200 << 30)
         "step_p50_ms": st["step_p50_ms"], "step_p99_ms": st["step_p99_ms"],
         "job_p50_ms": st["job_p50_ms"], "job_p99_ms": st["job_p99_ms"],
         "pairs_per_s": st["pairs"] / max(st["busy_ms"], 1e-9) * 1000.0,
-        "steps": st["steps"],
+        "steps": st["steps"], "n_range": [n_lo, n_hi], "steps_range": [s_lo, s_hi],
+        "still_running": st["running"] + st["queued"],
     }
+
+
+# ------------------------------------------------------------------ workloads
+def _drift(d0, d1):
+    out = {}
+    for k in ("total_vorticity", "linear_impulse"):
+        out[k] = float(np.linalg.norm(np.asarray(d1[k]) - np.asarray(d0[k])))
+    out["kinetic_energy_rel"] = float((d1["kinetic_energy"] - d0["kinetic_energy"]) /
+                                      d0["kinetic_energy"])
+    return out
+
+
+def bench_ring1(pa, gen, args):
+    """Single ring, 2,048 particles, binary64, RK4 dt 0.01 x 100 steps, and the
+    same run on the CPU oracle (reference-faithful OpenMP loop) for parity and
+    the CPU time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc  # CPU baseline / checker
+    w = gen.config_single_ring()
+    S = pa.Interaction(pa.Wake.from_dict(w), nu=w["nu"])
+    d0 = S.diagnose()
+    x0 = S.getStates()
+    t0 = time.perf_counter()
+    S.rk4(w["dt"], w["steps"])
+    gpu_s = time.perf_counter() - t0
+    d1 = S.diagnose()
+    xg = S.getStates()
+    S.close()
+    t0 = time.perf_counter()
+    xc = orc.rk4(x0, w["radius"], w["volume"], w["nu"], w["dt"], w["steps"], mode=orc.OMP_FAITHFUL)
+    cpu_s = time.perf_counter() - t0
+    n = w["n"]
+    pairs = 4.0 * w["steps"] * n * (n - 1)
+    err = float(np.max(np.abs(xg - xc)) / np.max(np.abs(xc)))
+    return {"metric": "single ring 2,048 particles, RK4 dt 0.01 x 100: pair-interactions/s",
+            "value": pairs / gpu_s, "unit": "pair-interactions/s", "n_gpus": 1,
+            "steps": w["steps"], "ms_per_step": gpu_s * 1e3 / w["steps"], "dtype": "f64",
+            "config": {"workload": "single_ring_2048", "dt": w["dt"], "nu": w["nu"]},
+            "trajectory_max_rel_diff_vs_cpu": err, "drift": _drift(d0, d1),
+            "cpu_baseline": {"value": pairs / cpu_s, "seconds": cpu_s, "cores": orc.max_threads(),
+                             "kind": "port", "sample": "the full 100-step run"}}
+
+
+def bench_leapfrog(pa, gen, args):
+    """Two coaxial rings, 2 x 32,768 particles, as two wakes; binary64 and
+    compensated binary32, RK4 dt 0.005 x steps (400 for the full config)."""
+    w = gen.config_leapfrog()
+    steps = args.steps
+    out = {"metric": "leapfrogging rings 65,536 particles: pair-interactions/s",
+           "unit": "pair-interactions/s", "n_gpus": 1, "steps": steps,
+           "config": {"workload": "leapfrog_65536", "wakes": w["wakes"], "dt": w["dt"],
+                      "nu": w["nu"]}}
+    n, n1 = w["n"], w["wakes"][0]
+    x0 = np.concatenate([w["pos"][:n1].ravel(), w["vort"][:n1].ravel(),
+                         w["pos"][n1:].ravel(), w["vort"][n1:].ravel()])
+    rates = {}
+    for name, prec in (("fp64", pa.FP64), ("fp32c", pa.FP32C)):
+        a = {k: w[k][:n1] for k in ("pos", "vort", "radius", "volume")}
+        b = {k: w[k][n1:] for k in ("pos", "vort", "radius", "volume")}
+        S = pa.Interaction(pa.Wake.from_dict(a), pa.Wake.from_dict(b), nu=w["nu"], precision=prec)
+        S.setStates(x0)
+        S.solve()
+        rates[name] = S.getRates()
+        d0 = S.diagnose()
+        t0 = time.perf_counter()
+        S.rk4(w["dt"], steps)
+        sec = time.perf_counter() - t0
+        d1 = S.diagnose()
+        S.close()
+        out[name] = {"value": 4.0 * steps * n * (n - 1) / sec, "ms_per_step": sec * 1e3 / steps,
+                     "drift": _drift(d0, d1)}
+    out["value"] = out["fp64"]["value"]
+    out["dtype"] = "f64"
+    r64, r32 = rates["fp64"], rates["fp32c"]
+    rr = lambda a, b: float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))  # noqa: E731
+    v = np.r_[r64[:3 * n1], r64[6 * n1:6 * n1 + 3 * (n - n1)]]
+    v32 = np.r_[r32[:3 * n1], r32[6 * n1:6 * n1 + 3 * (n - n1)]]
+    out["fp32c"]["rel_rms_rate_err_vs_fp64_t0"] = {"all": rr(r32, r64), "velocity": rr(v32, v)}
+    return out
+
+
+def bench_rotor_growth(pa, gen, args):
+    """Rotor wake grown by shedding: each step emits 256 particles, then one
+    RK4 step (dt 1e-3); compensated binary32 pairs, binary64 state."""
+    shed = args.shed_steps
+    e0 = gen.rotor_emission(0)
+    S = pa.Interaction(pa.Wake.from_dict(e0), nu=1e-4, precision=pa.FP32C, uinf=(0, 0, -0.05))
+    pairs, t0 = 0.0, time.perf_counter()
+    n = e0["n"]
+    for k in range(shed):
+        if k > 0:
+            e = gen.rotor_emission(k)
+            S.appendParticles(0, e["pos"], e["vort"], e["radius"], e["volume"])
+            n += e["n"]
+        S.rk4(STEP_DT, 1)
+        pairs += 4.0 * n * (n - 1)
+        if k % 128 == 0:
+            log(f"  rotor growth step {k} N={n}")
+    sec = time.perf_counter() - t0
+    d = S.diagnose(ke=False)
+    S.close()
+    return {"metric": "rotor wake grown to N particles, FP32c pairs, FP64 state: pair-interactions/s",
+            "value": pairs / sec, "unit": "pair-interactions/s", "n_gpus": 1, "steps": shed,
+            "dtype": "f32c", "config": {"workload": f"rotor_growth_{n}", "final_n": n,
+                                        "dt": STEP_DT}, "seconds": sec,
+            "total_vorticity": d["total_vorticity"].tolist()}
+
+
+def bench_batch(pa, gen, args):
+    """256 independent rings (N log-uniform in [1,024, 65,536]) stepped
+    together: one persistent launch per stage over the job table, HIP-graph
+    replay of the step, variant picked by timing candidates on the batch."""
+    jobs = gen.batch_jobs(args.jobs, seed=0)
+    pairs_rhs = sum(float(j["n"]) * (j["n"] - 1) for j in jobs)
+    out = {"metric": f"{len(jobs)} batched ring simulations: pair-interactions/s",
+           "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,
+           "config": {"workload": f"batch_{len(jobs)}_rings", "particles": sum(j["n"] for j in jobs),
+                      "pairs_per_rhs": pairs_rhs, "dt": 0.005}}
+    for name, prec in (("fp64", pa.FP64), ("fp32c", pa.FP32C)):
+        best = None
+        for block in (64, 128, 256):
+            for tpt in (1, 2):
+                B = pa.Batch(block=block, targets_per_thread=tpt)
+                for j in jobs:
+                    B.add(j, j["nu"], j["dt"], prec)
+                B.step(1, graph=True)
+                t0 = time.perf_counter()
+                B.step(1, graph=True)
+                sec = time.perf_counter() - t0
+                B.close()
+                log(f"  batch {name} block {block} tpt {tpt}: {sec * 1e3:.1f} ms/step")
+                if best is None or sec < best[0]:
+                    best = (sec, block, tpt)
+        B = pa.Batch(block=best[1], targets_per_thread=best[2])
+        ids = [B.add(j, j["nu"], j["dt"], prec) for j in jobs]
+        B.step(args.warmup, graph=True)
+        t0 = time.perf_counter()
+        B.step(args.steps, graph=True)
+        sec = time.perf_counter() - t0
+        bad = sum(1 for i in ids if B.status(i) != 0)
+        B.close()
+        out[name] = {"value": 4.0 * args.steps * pairs_rhs / sec, "ms_per_step": sec * 1e3 / args.steps,
+                     "variant": {"block": best[1], "targets_per_thread": best[2]},
+                     "failed_jobs": bad}
+    out["value"] = out["fp64"]["value"]
+    out["dtype"] = "f64"
+    out["fp32c"]["speedup_vs_fp64"] = out["fp32c"]["value"] / out["fp64"]["value"]
+    return out
+
+
+def bench_serve(pa, gen, args):
+    """Serving stream: Poisson arrivals at --rate per second for
+    --serve-seconds; each job a ring or rotor wake with N log-uniform in
+    [2,048, 262,144], steps uniform in [50, 500], FP64 or FP32c 50/50, 200 GB
+    HBM budget."""
+    r = serving_run(pa, gen, duration_s=args.serve_seconds, rate_hz=args.rate, n_lo=2048,
+                    n_hi=262144, s_lo=50, s_hi=500, drain_factor=args.drain)
+    return {"metric": "serving: p50/p99 step and job latency, aggregate pair-interactions/s",
+            "value": r["pairs_per_s"], "unit": "pair-interactions/s", "n_gpus": 1, "dtype": "mixed",
+            "config": {"workload": "serving_stream", "n_range": [2048, 262144],
+                       "steps_range": [50, 500], "hbm_budget_gb": 200}, "serving": r}
 
 
 def main():
@@ -182,6 +346,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner; default variant")
+    ap.add_argument("--workload", default="rotor262k",
+                    choices=["rotor262k", "ring1", "leapfrog", "rotor_growth", "batch", "serve"])
+    ap.add_argument("--shed-steps", type=int, default=1024)
+    ap.add_argument("--jobs", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=2.0, help="serving arrivals per second")
+    ap.add_argument("--drain", type=float, default=3.0, help="serving: stop after drain x duration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -190,6 +360,12 @@ def main():
 
     import pawan_amd as pa
     from pawan_amd import generators as gen
+
+    other = {"ring1": bench_ring1, "leapfrog": bench_leapfrog, "rotor_growth": bench_rotor_growth,
+             "batch": bench_batch, "serve": bench_serve}
+    if args.workload in other:
+        print(json.dumps(other[args.workload](pa, gen, args)), flush=True)
+        return
 
     w = gen.config_rotor_wake(args.n_shed)
     n = w["n"]
