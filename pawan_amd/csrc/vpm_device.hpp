@@ -39,6 +39,14 @@ __device__ __forceinline__ double rsq_nr(double t) {
     return __builtin_fma(r * 0.5, e, r);
 }
 
+// Twice the reciprocal square root, from the same estimate and Newton step
+// written as 2r' = r (3 - t r^2): three operations instead of four.  The
+// factor 2 is exact; the pair kernels fold it into their constants.
+__device__ __forceinline__ double rsq2_nr(double t) {
+    const double r = __builtin_amdgcn_rsq(t);
+    return r * __builtin_fma(-t, r * r, 3.0);
+}
+
 template <int KIND>
 __device__ __forceinline__ void kernel64(double t, double rt, double &Q, double &Fp, double &Zs) {
     const double it = rt * rt;
@@ -59,6 +67,26 @@ __device__ __forceinline__ void kernel64(double t, double rt, double &Q, double 
     Fp = __builtin_fma(-3.0, Q, Zs) * it;
 }
 
+// HIGHORDER kernel for the binary64 pair pass, scaled: returns kq Q, kq F'
+// and kz Zs with kz = 2^15 / (1.875/pi), kq = 4 kz (pair_scale), from
+// R = 2 t^-1/2.  Every power-of-two factor is exact and the 1.875/pi of Zs
+// moves into the epilogue, which saves three operations per pair.
+__device__ __forceinline__ void kernel64_scaled(double t, double R, double &Q, double &Fp,
+                                                double &Zs) {
+    const double W = rsq2_nr(t + 1.0);  // 2 (t+1)^-1/2
+    const double iu = W * W;
+    const double iu2 = iu * iu;         // 16 (t+1)^-2
+    const double w5 = iu2 * W;          // 32 (t+1)^-2.5
+    Zs = w5 * w5 * w5;                  // 2^15 (t+1)^-7.5 = kz Zs
+    // kz [(0.25/pi)(t+1.5)(t+1)^-2 t^-1/2 - Zs] with iu2 R = 32 (t+1)^-2 t^-1/2
+    const double P = __builtin_fma(__builtin_fma(t, K_HC64, 1.5 * K_HC64), iu2 * R, -Zs);
+    const double it = R * R;            // 4 / t
+    Q = P * it;                         // kq Q
+    Fp = __builtin_fma(-0.75, Q, Zs) * it;  // 4 kz (Zs - 3 Q) / t = kq F'
+}
+
+// Binary32 kernel.  HIGHORDER is scaled by kz = kq = pi/1.875 (pair_scale):
+// the 1.875/pi of Zs moves into the epilogue.
 template <int KIND>
 __device__ __forceinline__ void kernel32(float t, float rt, float &Q, float &Fp, float &Zs) {
     const float it = rt * rt;
@@ -67,9 +95,9 @@ __device__ __forceinline__ void kernel32(float t, float rt, float &Q, float &Fp,
         const float iu = w * w;
         const float iu2 = iu * iu;
         const float w5 = iu2 * w;
-        Zs = (float)K_CZ * (w5 * w5 * w5);
+        Zs = w5 * w5 * w5;
         const float P =
-            __builtin_fmaf(__builtin_fmaf(t, (float)K_CPHI, (float)(1.5 * K_CPHI)), iu2 * rt, -Zs);
+            __builtin_fmaf(__builtin_fmaf(t, (float)K_HC32, (float)(1.5 * K_HC32)), iu2 * rt, -Zs);
         Q = P * it;
     } else {
         Zs = (float)K_CG * __expf(-0.5f * t);
@@ -94,12 +122,18 @@ __device__ __forceinline__ void pair64(double xi, double yi, double zi, double a
     const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
     double Q, Fp, Zs;
     if (UNIFORM) {
-        kernel64<KIND>(r2, rsq_nr(r2), Q, Fp, Zs);
+        if (KIND == KIND_HIGHORDER)
+            kernel64_scaled(r2, rsq2_nr(r2), Q, Fp, Zs);
+        else
+            kernel64<KIND>(r2, rsq_nr(r2), Q, Fp, Zs);
     } else {
         const double isg = rsq_nr(hi + s.h);  // 1/sigma
         const double is2 = isg * isg;
         const double t = r2 * is2;
-        kernel64<KIND>(t, rsq_nr(t), Q, Fp, Zs);
+        if (KIND == KIND_HIGHORDER)
+            kernel64_scaled(t, rsq2_nr(t), Q, Fp, Zs);
+        else
+            kernel64<KIND>(t, rsq_nr(t), Q, Fp, Zs);
         const double s3 = isg * is2, s5 = s3 * is2;
         Q *= s3;
         Fp *= s5;
@@ -408,9 +442,10 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
 // rates
 //   u   = su * S[0:3] + uinf
 //   da  = sq * S[3:6] + sf * S[6:9] + sd * (v_i S[9:12] - a_i S[12])
-// (uniform core, positions prescaled by 1/sigma: su = sigma^-2,
-// sq = sf = sigma^-3, sd = 2 nu sigma^-5; otherwise the per-pair factors are
-// already inside and su = sq = sf = 1, sd = 2 nu), then apply the RK4 / Euler stage combination of rk4.cpp:21-66
+// (uniform core, positions prescaled by 1/sigma: su = sigma^-2 / kq,
+// sq = sf = sigma^-3 / kq, sd = 2 nu sigma^-5 / kz; otherwise the per-pair
+// powers of sigma are already inside and su = sq = sf = 1/kq, sd = 2 nu / kz;
+// kq, kz from pair_scale), then apply the RK4 / Euler stage combination of rk4.cpp:21-66
 // with unfused multiplies and adds in the reference's order, and write the
 // next stage's record.  Block 0 also re-arms the pair kernels' queues.
 template <typename Rec>

@@ -134,6 +134,9 @@ JobDev Job::dev() const {
     d.nu = par.nu;
     for (int c = 0; c < 3; ++c) d.uinf[c] = par.uinf[c];
     d.rec_scale = rec_scale();
+    // the pair kernels' sums carry kq (Q, F') and kz (Zs)
+    const PairScale ps = pair_scale(kind, par.precision);
+    const long double kq = ps.q, kz = ps.z;
     if (uniform) {
         // sigma^2 = (s^2 + s^2)/2 = s^2; records hold x / sigma, so
         // d = sigma d^, q = sigma^-3 Q, F = sigma^-5 F', Z = sigma^-3 Zs
@@ -141,14 +144,14 @@ JobDev Job::dev() const {
         long double sg = sqrtl(s2);
         long double s3 = s2 * sg, s5 = s3 * s2;
         d.inv_s2 = 1.0;
-        d.su = (double)(1.0L / s2);
-        d.sq = (double)(1.0L / s3);
+        d.su = (double)(1.0L / (s2 * kq));
+        d.sq = (double)(1.0L / (s3 * kq));
         d.sf = d.sq;
-        d.sd = (double)(2.0L * par.nu / s5);
+        d.sd = (double)(2.0L * par.nu / (s5 * kz));
     } else {
         d.inv_s2 = 0.0;
-        d.su = d.sq = d.sf = 1.0;
-        d.sd = 2.0 * par.nu;
+        d.su = d.sq = d.sf = (double)(1.0L / kq);
+        d.sd = (double)(2.0L * par.nu / kz);
     }
     return d;
 }
@@ -256,6 +259,7 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     j->n = n;
     j->npad = round_up(n, 256);
     j->par = par;
+    j->kind = kind_;
     size_t L = chunk_override ? chunk_override : chunk_for(n, tb_);
     if (L % (size_t)tb_ != 0) throw Error(PW_EINVAL, "chunk must be a multiple of the block size");
     j->chunk_len = (int)L;

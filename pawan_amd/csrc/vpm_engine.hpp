@@ -75,6 +75,7 @@ struct Job {
     size_t n = 0, npad = 0;
     int nchunk = 0, chunk_len = 0, ntile = 0;
     JobParams par;
+    int kind = 0;  // smoothing kernel of the engine that owns the job
     bool uniform = false;
     double sigma = 0.0;
     DevBuf rec, soa, part, vol, flag;  // soa holds x0, xs, k1, k23, rate

@@ -27,6 +27,25 @@ constexpr double K_PI = 3.14159265358979323846;
 constexpr double K_CZ = 1.875 / K_PI;   // HIGHORDER zeta(0)
 constexpr double K_CPHI = 0.25 / K_PI;  // phi prefactor
 constexpr double K_CG = 0.063493635934240969;  // (2 pi)^-1.5
+// phi prefactor of the scaled HIGHORDER kernels: kz K_CPHI / 32 for binary64
+// (kz = 2^15 / K_CZ), kz K_CPHI for binary32 (kz = 1 / K_CZ)
+constexpr double K_HC64 = 2048.0 / 15.0;
+constexpr double K_HC32 = 2.0 / 15.0;
+
+// Factors the pair kernels leave on their sums: kq on Q and F' (velocity,
+// stretching, d(d.e) terms), kz on Zs (PSE sums).  The epilogue divides them
+// out.  Host and device code.
+struct PairScale {
+    double q, z;
+};
+__host__ __device__ inline PairScale pair_scale(int kind, int precision) {
+    if (kind != KIND_HIGHORDER) return PairScale{1.0, 1.0};
+    if (precision == PREC_FP64) {
+        const double kz = 32768.0 / K_CZ;
+        return PairScale{4.0 * kz, kz};
+    }
+    return PairScale{1.0 / K_CZ, 1.0 / K_CZ};
+}
 constexpr double K_SQRT1_2 = 0.70710678118654752440;
 
 // Source/target record of the binary64 path: 64 bytes, read as 4 x 16 B.
