@@ -55,6 +55,15 @@ extern "C" {
 #define PW_FP32C 1 /* binary32 pair terms, compensated per-target sums,
                       binary64 cross-tile reduction, state and diagnostics    */
 
+/* how the binary32 pass visits pairs (binary64 always uses one-sided) */
+#define PW_SCHEME_DEFAULT 0   /* symmetric                                    */
+#define PW_SCHEME_ONESIDED 1  /* every ordered pair, per target (no source-side
+                                 sums)                                         */
+#define PW_SCHEME_SYMMETRIC 2 /* every unordered pair once, both particles
+                                 updated, as the reference's INTERACT does
+                                 (interaction_utils.h:429-481); deterministic
+                                 per-block source-side slots                   */
+
 typedef struct pw_config {
     int device;       /* HIP device ordinal                                      */
     int kernel;       /* PW_KERNEL_*                                             */
@@ -65,6 +74,7 @@ typedef struct pw_config {
     int chunk;        /* sources per work item (multiple of block); 0 = policy   */
     int targets_per_thread; /* 1 or 2; 0 = tuned or default                      */
     int unroll;       /* source-loop unroll 1/2/4; 0 = tuned or default          */
+    int scheme;       /* PW_SCHEME_*: pair visiting of the binary32 pass         */
 } pw_config;
 
 /* A compiled kernel variant of the pair pass. */
@@ -72,6 +82,7 @@ typedef struct pw_variant {
     int block;              /* 64, 128 or 256 threads per workgroup               */
     int targets_per_thread; /* 1 or 2                                             */
     int unroll;             /* 1, 2 or 4; applies to the precision being tuned    */
+    int scheme;             /* PW_SCHEME_* of the binary32 pass (0: default)       */
 } pw_variant;
 
 typedef struct pw_ctx pw_ctx;
@@ -235,17 +246,22 @@ int pw_server_job_state(pw_server *s, int64_t id, int64_t *steps_done);
 /* final positions/vorticities of a finished job (N x 3 each); frees it */
 int pw_server_fetch(pw_server *s, int64_t id, double *pos, double *vort);
 int pw_server_get_stats(pw_server *s, pw_server_stats *st);
-/* device bytes one job of n particles occupies while resident */
+/* device bytes one job of n particles occupies while resident, for the
+ * default variant of a context of n particles (a binary32 job includes the
+ * symmetric pass's source-side slots, 40 n^2 / (block x targets per thread)
+ * bytes) */
 size_t pw_job_footprint(size_t n, int precision);
 
 /* -------------------------------------------------------------- autotuner
- * Times every compiled variant (block x targets per thread x unroll) of the
- * pair pass on the given particles (reps solves each, HIP events) and returns
- * them in `all` (up to max_all entries) and the fastest in *best.  Results
- * are bitwise identical across variants: only the speed differs. */
+ * Times every compiled variant (block x targets per thread x unroll, and for
+ * PW_FP32C both pair schemes) on the given particles (reps solves each, HIP
+ * events) and returns them in `all` (up to max_all entries) and the fastest
+ * in *best.  One-sided variants give bitwise identical results; symmetric
+ * ones group their sums by block and agree to binary32 rounding. */
 typedef struct pw_tune_entry {
     pw_variant variant;
-    double pair_ms;      /* mean pair-kernel time per solve       */
+    double pair_ms;      /* mean device time per solve: pair pass and
+                            reduction (HIP events)                  */
     double pairs_per_s;  /* ordered pair-interactions per second   */
 } pw_tune_entry;
 

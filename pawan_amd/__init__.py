@@ -7,8 +7,10 @@ ABI, include/pawan_hip.h); this package is its host-side mirror of the
 reference's classes plus the seeded workload generators.
 """
 from . import generators  # noqa: F401
-from ._lib import FP32C, FP64, KERNEL_GAUSSIAN, KERNEL_HIGHORDER, PawanError  # noqa: F401
+from ._lib import (FP32C, FP64, KERNEL_GAUSSIAN, KERNEL_HIGHORDER, SCHEME_DEFAULT,  # noqa: F401
+                   SCHEME_ONESIDED, SCHEME_SYMMETRIC, PawanError)
 from .interaction import RK4, Batch, Integration, Interaction, Server, Wake, autotune  # noqa: F401
 
 __all__ = ["Interaction", "Integration", "RK4", "Batch", "Server", "Wake", "autotune", "generators",
-           "FP64", "FP32C", "KERNEL_HIGHORDER", "KERNEL_GAUSSIAN", "PawanError"]
+           "FP64", "FP32C", "KERNEL_HIGHORDER", "KERNEL_GAUSSIAN", "SCHEME_DEFAULT",
+           "SCHEME_ONESIDED", "SCHEME_SYMMETRIC", "PawanError"]

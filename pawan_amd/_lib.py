@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libpawan_hip.so")
 PW_OK, PW_EINVAL, PW_ENOMEM, PW_EHIP, PW_ENONFINITE, PW_ENOTFOUND = 0, 1, 2, 3, 4, 5
 KERNEL_HIGHORDER, KERNEL_GAUSSIAN = 0, 1
 FP64, FP32C = 0, 1
+SCHEME_DEFAULT, SCHEME_ONESIDED, SCHEME_SYMMETRIC = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 
@@ -22,11 +23,13 @@ _dp = C.POINTER(C.c_double)
 class PwConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("kernel", C.c_int), ("precision", C.c_int),
                 ("nu", C.c_double), ("uinf", C.c_double * 3), ("block", C.c_int),
-                ("chunk", C.c_int), ("targets_per_thread", C.c_int), ("unroll", C.c_int)]
+                ("chunk", C.c_int), ("targets_per_thread", C.c_int), ("unroll", C.c_int),
+                ("scheme", C.c_int)]
 
 
 class PwVariant(C.Structure):
-    _fields_ = [("block", C.c_int), ("targets_per_thread", C.c_int), ("unroll", C.c_int)]
+    _fields_ = [("block", C.c_int), ("targets_per_thread", C.c_int), ("unroll", C.c_int),
+                ("scheme", C.c_int)]
 
 
 class PwTuneEntry(C.Structure):

@@ -50,6 +50,13 @@ int default_block(size_t n) {
     return 64;
 }
 
+// Two targets per thread for large binary32 jobs: the symmetric pass's
+// source-side slots take 40 N^2 / (block x targets per thread) bytes.  (The
+// GAUSSIAN kernel is compiled with one target per thread only.)
+int default_tpt(size_t n, int precision, int kernel) {
+    return kernel == PW_KERNEL_HIGHORDER && precision == PW_FP32C && n >= 32768 ? 2 : 1;
+}
+
 bool finite_params(double nu, double dt, const double *u) {
     return std::isfinite(nu) && std::isfinite(dt) && std::isfinite(u[0]) && std::isfinite(u[1]) &&
            std::isfinite(u[2]);
@@ -75,18 +82,21 @@ struct pw_ctx {
     pw::Variant choose_variant() const {
         pw::Variant v;
         v.block = cfg.block ? cfg.block : default_block(total());
-        v.tpt = cfg.targets_per_thread ? cfg.targets_per_thread : 1;
+        v.tpt = cfg.targets_per_thread ? cfg.targets_per_thread : default_tpt(total(), cfg.precision, cfg.kernel);
         if (cfg.unroll) v.unroll64 = v.unroll32 = cfg.unroll;
+        v.sym32 = cfg.scheme != PW_SCHEME_ONESIDED;
         const char *cache = std::getenv("PW_TUNE_CACHE");
-        if (cache && !cfg.block && !cfg.targets_per_thread && !cfg.unroll) {
+        if (cache && !cfg.block && !cfg.targets_per_thread && !cfg.unroll && !cfg.scheme) {
             pw_variant t;
             if (pw_tune_lookup(cache, cfg.kernel, cfg.precision, total(), &t) == PW_OK) {
                 v.block = t.block;
                 v.tpt = t.targets_per_thread;
-                if (cfg.precision == PW_FP64)
+                if (cfg.precision == PW_FP64) {
                     v.unroll64 = t.unroll;
-                else
-                    v.unroll32 = t.unroll;
+                } else {
+                    v.sym32 = t.scheme != PW_SCHEME_ONESIDED;
+                    if (t.unroll) v.unroll32 = t.unroll;
+                }
             }
         }
         return v;
@@ -97,7 +107,7 @@ struct pw_ctx {
         pw::Variant v = choose_variant();
         pw::Variant cur = eng ? eng->variant() : pw::Variant{};
         if (!eng || cur.block != v.block || cur.tpt != v.tpt || cur.unroll64 != v.unroll64 ||
-            cur.unroll32 != v.unroll32) {
+            cur.unroll32 != v.unroll32 || cur.sym32 != v.sym32) {
             job = nullptr;
             eng.reset();
             eng.reset(new Engine(cfg.device, cfg.kernel, v));
@@ -164,6 +174,8 @@ int pw_create(pw_ctx **out, const pw_config *cfg) {
             throw Error(PW_EINVAL, "targets_per_thread must be 0, 1 or 2");
         if (c.unroll != 0 && c.unroll != 1 && c.unroll != 2 && c.unroll != 4)
             throw Error(PW_EINVAL, "unroll must be 0, 1, 2 or 4");
+        if (c.scheme < PW_SCHEME_DEFAULT || c.scheme > PW_SCHEME_SYMMETRIC)
+            throw Error(PW_EINVAL, "unknown pair scheme");
         if (!finite_params(c.nu, 0.0, c.uinf)) throw Error(PW_EINVAL, "non-finite parameter");
         int ndev = 0;
         pw::hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
@@ -424,8 +436,11 @@ extern "C" {
 static pw::Variant variant_of(const pw_variant *v, int precision_hint) {
     pw::Variant r;
     if (!v) return r;
+    if (v->scheme < PW_SCHEME_DEFAULT || v->scheme > PW_SCHEME_SYMMETRIC)
+        throw Error(PW_EINVAL, "unknown pair scheme");
     r.block = v->block ? v->block : 128;
     r.tpt = v->targets_per_thread ? v->targets_per_thread : 1;
+    r.sym32 = v->scheme != PW_SCHEME_ONESIDED;
     if (v->unroll) {
         if (precision_hint == PW_FP32C)
             r.unroll32 = v->unroll;
@@ -565,7 +580,10 @@ struct pw_server {
 
 extern "C" {
 
-size_t pw_job_footprint(size_t n, int precision) { return pw::job_footprint(n, precision, 128); }
+size_t pw_job_footprint(size_t n, int precision) {
+    return pw::job_footprint(n, precision, default_block(n),
+                             default_tpt(n, precision, PW_KERNEL_HIGHORDER), true);
+}
 
 int pw_server_create(pw_server **out, int device, int kernel, size_t hbm_budget_bytes,
                      const pw_variant *variant) {
@@ -602,7 +620,8 @@ int pw_server_submit(pw_server *s, const pw_job_spec *spec, size_t nsteps, doubl
         j.id = s->next_id++;
         j.n = spec->n;
         j.nsteps = nsteps;
-        j.bytes = pw::job_footprint(spec->n, spec->precision, s->eng->block());
+        const pw::Variant ev = s->eng->variant();
+        j.bytes = pw::job_footprint(spec->n, spec->precision, ev.block, ev.tpt, ev.sym32 != 0);
         if (j.bytes > s->budget) throw Error(PW_ENOMEM, "job larger than the HBM budget");
         j.arrival = arrival_ms < 0 ? s->now() : arrival_ms;
         j.pos.assign(spec->pos, spec->pos + 3 * spec->n);
@@ -744,18 +763,27 @@ int pw_autotune(int device, int kernel, int precision, size_t n, const double *p
     return guarded(nullptr, [&] {
         if (precision != PW_FP64 && precision != PW_FP32C) throw Error(PW_EINVAL, "precision");
         const int blocks[3] = {64, 128, 256}, tpts[2] = {1, 2}, unrs[3] = {1, 2, 4};
+        // binary32: the symmetric pass (no unroll parameter) and the one-sided one
+        std::vector<int> schemes;
+        if (precision == PW_FP32C) schemes = {PW_SCHEME_SYMMETRIC, PW_SCHEME_ONESIDED};
+        else schemes = {PW_SCHEME_ONESIDED};
         std::vector<pw_tune_entry> res;
+        for (int sc : schemes)
         for (int b : blocks)
             for (int t : tpts)
                 for (int u : unrs) {
+                    const bool sym = sc == PW_SCHEME_SYMMETRIC;
+                    if (sym && u != unrs[0]) continue;
                     pw::Variant v;
                     v.block = b;
                     v.tpt = t;
+                    v.sym32 = sym ? 1 : 0;
                     if (precision == PW_FP64)
                         v.unroll64 = u;
-                    else
+                    else if (!sym)
                         v.unroll32 = u;
-                    if (!pw::variant_exists(kernel, v.block, v.tpt, v.unroll64, v.unroll32)) continue;
+                    if (!pw::variant_exists(kernel, v.block, v.tpt, v.unroll64, v.unroll32, v.sym32))
+                        continue;
                     Engine eng(device, kernel, v);
                     JobParams par;
                     par.precision = precision;
@@ -765,12 +793,13 @@ int pw_autotune(int device, int kernel, int precision, size_t n, const double *p
                     const int r = std::max(1, reps);
                     for (int k = 0; k < r; ++k) {
                         eng.solve();
-                        ms += eng.last_pair_ms;
+                        ms += eng.last_total_ms;  // pair pass + reduction
                     }
                     pw_tune_entry e;
                     e.variant.block = b;
                     e.variant.targets_per_thread = t;
-                    e.variant.unroll = u;
+                    e.variant.unroll = sym ? 0 : u;
+                    e.variant.scheme = sc;
                     e.pair_ms = ms / r;
                     e.pairs_per_s = (double)n * (double)(n - 1) / (e.pair_ms * 1e-3);
                     res.push_back(e);
@@ -811,7 +840,8 @@ int pw_tune_save(const char *path, int kernel, int precision, size_t n, const pw
         if (!out) throw Error(PW_EINVAL, "cannot write the tuning cache");
         for (auto &l : keep) out << l << "\n";
         out << kernel << " " << precision << " " << bk << " " << e->variant.block << " "
-            << e->variant.targets_per_thread << " " << e->variant.unroll << " " << e->pair_ms << "\n";
+            << e->variant.targets_per_thread << " " << e->variant.unroll << " "
+            << e->variant.scheme << " " << e->pair_ms << "\n";
     });
 }
 
@@ -823,14 +853,18 @@ int pw_tune_lookup(const char *path, int kernel, int precision, size_t n, pw_var
     std::string line;
     while (std::getline(in, line)) {
         std::istringstream ss(line);
-        int k, p, b, blk, tpt, unr;
-        if (ss >> k >> p >> b >> blk >> tpt >> unr && k == kernel && p == precision && b == bk) {
-            if (!pw::variant_exists(kernel, blk, tpt, precision == PW_FP64 ? unr : 2,
-                                    precision == PW_FP32C ? unr : 4))
+        int k, p, b, blk, tpt, unr, sc;
+        if (ss >> k >> p >> b >> blk >> tpt >> unr >> sc && k == kernel && p == precision &&
+            b == bk) {
+            const int sym = precision == PW_FP32C && sc != PW_SCHEME_ONESIDED;
+            if (sc < PW_SCHEME_DEFAULT || sc > PW_SCHEME_SYMMETRIC ||
+                !pw::variant_exists(kernel, blk, tpt, precision == PW_FP64 ? unr : 2,
+                                    precision == PW_FP32C && unr ? unr : 4, sym))
                 return PW_EINVAL;
             out->block = blk;
             out->targets_per_thread = tpt;
             out->unroll = unr;
+            out->scheme = sc;
             return PW_OK;
         }
     }

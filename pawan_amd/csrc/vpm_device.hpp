@@ -437,6 +437,341 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
     }
 }
 
+// ------------------------------------------- symmetric binary32 pair pass
+// Each unordered pair (i < j) is evaluated once: the kernel values, the
+// displacement and the stretching terms are shared by both particles, which
+// saves about a third of the arithmetic per ordered pair.  A workgroup owns a
+// block of TB x TPT targets (thread t holds targets t, t + TB, ...) and walks
+// the sources j > i of one chunk, 64 at a time, staged through LDS as
+// structure of arrays.  Within a 64-source tile, lane l takes source
+// (l + s) mod 64 at step s, so the 64 lanes meet 64 different sources at
+// every step and each source meets every lane once in 64 steps.  The source-
+// side sums ride along: each step they move one lane down (DPP wave rotate,
+// fused into the add where the compiler can), so lane l always holds the
+// sums of the source it is working on.  (LDS float atomics, the other way to
+// keep per-source sums, measured 12x slower on gfx950.)  Both sides are
+// binary32 runs of PW_SYM_RUN terms added into binary64: the target side in
+// registers, then the (target, chunk) slot; the source side in LDS, then
+// added over the waves in wave order and written once to the (source, block)
+// slot.  Every slot has one writer and every sum a fixed order, so results do
+// not depend on scheduling.
+
+// Terms per binary32 run before it is added into binary64 (both sides).
+// Shorter runs bound the rounding of the near-field terms, which in this
+// pass land on different sides for a particle's neighbours before and after
+// it and cancel only in binary64: 32 terms cost 2x the one-sided pass's
+// stretching error on the rotor wake, 16 terms match it.
+#ifndef PW_SYM_RUN
+#define PW_SYM_RUN 16
+#endif
+
+
+struct Acc32s {
+    float v[NCOMP_S];
+};
+
+// x of lane (l + 1) mod 64, added to y (v_add_f32 with a wave_rol:1 operand).
+__device__ __forceinline__ float rot_add(float x, float y) {
+    return __builtin_bit_cast(float,
+                              __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x134, 0xf, 0xf, true)) +
+           y;
+}
+
+// A source tile in LDS, one array per field, so that 64 lanes reading 64
+// different sources hit 64 consecutive words.  Every source is stored twice
+// (entries k and k + 64), so lane l reads source (l + s) mod 64 at entry
+// l + s: within an unrolled run of steps the address is one register plus
+// an immediate.
+struct SrcTile {
+    double x[128], y[128], z[128], ax[128], ay[128], az[128];
+    float fx[128], fy[128], fz[128], v[128], h[128];
+};
+
+// Fields of one particle in registers.
+struct Part32 {
+    double x, y, z, ax, ay, az;
+    float fx, fy, fz, v, h;
+};
+
+__device__ __forceinline__ Part32 load_src(const SrcTile &t, int k) {
+    Part32 p;
+    p.x = t.x[k];
+    p.y = t.y[k];
+    p.z = t.z[k];
+    p.ax = t.ax[k];
+    p.ay = t.ay[k];
+    p.az = t.az[k];
+    p.fx = t.fx[k];
+    p.fy = t.fy[k];
+    p.fz = t.fz[k];
+    p.v = t.v[k];
+    p.h = t.h[k];
+    return p;
+}
+
+// One unordered pair (target me, source s): target-side terms added to a,
+// source-side terms stored (FIRST) or added to b.  MASKED: a pair with
+// valid == false contributes exactly zero (its operands are finite).
+template <int KIND, bool UNIFORM, bool MASKED, bool FIRST>
+__device__ __forceinline__ void pair32s(const Part32 &me, const Part32 &s, bool valid, Acc32s &a,
+                                        Acc32s &b) {
+    const float dx = (float)(me.x - s.x), dy = (float)(me.y - s.y), dz = (float)(me.z - s.z);
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    float Q, Fp, Zs;
+    if (UNIFORM) {
+        const float t = MASKED ? (valid ? r2 : 1.0f) : r2;
+        kernel32<KIND>(t, __builtin_amdgcn_rsqf(t), Q, Fp, Zs);
+    } else {
+        const float isg = __builtin_amdgcn_rsqf(me.h + s.h);
+        const float is2 = isg * isg;
+        float t = r2 * is2;
+        if (MASKED) t = valid ? t : 1.0f;
+        kernel32<KIND>(t, __builtin_amdgcn_rsqf(t), Q, Fp, Zs);
+        const float s3 = isg * is2, s5 = s3 * is2;
+        Q *= s3;
+        Fp *= s5;
+        Zs *= s5;
+    }
+    if (MASKED) {
+        Q = valid ? Q : 0.0f;
+        Fp = valid ? Fp : 0.0f;
+        Zs = valid ? Zs : 0.0f;
+    }
+    // velocity of i: q (a_j x d); of j: q (a_i x (x_j - x_i)) = q (d x a_i)
+    const float cx = __builtin_fmaf(s.fy, dz, -s.fz * dy);
+    const float cy = __builtin_fmaf(s.fz, dx, -s.fx * dz);
+    const float cz = __builtin_fmaf(s.fx, dy, -s.fy * dx);
+    a.v[0] = __builtin_fmaf(Q, cx, a.v[0]);
+    a.v[1] = __builtin_fmaf(Q, cy, a.v[1]);
+    a.v[2] = __builtin_fmaf(Q, cz, a.v[2]);
+    const float gx = __builtin_fmaf(dy, me.fz, -dz * me.fy);
+    const float gy = __builtin_fmaf(dz, me.fx, -dx * me.fz);
+    const float gz = __builtin_fmaf(dx, me.fy, -dy * me.fx);
+    // stretching of i: q e + F (d.e) d with e = a_i x a_j formed in binary64;
+    // of j: the same with e and d negated, i.e. minus it
+    const float ex = (float)__builtin_fma(me.ay, s.az, -me.az * s.ay);
+    const float ey = (float)__builtin_fma(me.az, s.ax, -me.ax * s.az);
+    const float ez = (float)__builtin_fma(me.ax, s.ay, -me.ay * s.ax);
+    const float fd = Fp * __builtin_fmaf(dz, ez, __builtin_fmaf(dy, ey, dx * ex));
+    const float sx = __builtin_fmaf(fd, dx, Q * ex);
+    const float sy = __builtin_fmaf(fd, dy, Q * ey);
+    const float sz = __builtin_fmaf(fd, dz, Q * ez);
+    a.v[3] += sx;
+    a.v[4] += sy;
+    a.v[5] += sz;
+    // PSE sums: i gets k a_j, k v_j; j gets k a_i, k v_i
+    a.v[6] = __builtin_fmaf(Zs, s.fx, a.v[6]);
+    a.v[7] = __builtin_fmaf(Zs, s.fy, a.v[7]);
+    a.v[8] = __builtin_fmaf(Zs, s.fz, a.v[8]);
+    a.v[9] = __builtin_fmaf(Zs, s.v, a.v[9]);
+    if (FIRST) {
+        b.v[0] = Q * gx;
+        b.v[1] = Q * gy;
+        b.v[2] = Q * gz;
+        b.v[3] = sx;
+        b.v[4] = sy;
+        b.v[5] = sz;
+        b.v[6] = Zs * me.fx;
+        b.v[7] = Zs * me.fy;
+        b.v[8] = Zs * me.fz;
+        b.v[9] = Zs * me.v;
+    } else {
+        b.v[0] = __builtin_fmaf(Q, gx, b.v[0]);
+        b.v[1] = __builtin_fmaf(Q, gy, b.v[1]);
+        b.v[2] = __builtin_fmaf(Q, gz, b.v[2]);
+        b.v[3] += sx;
+        b.v[4] += sy;
+        b.v[5] += sz;
+        b.v[6] = __builtin_fmaf(Zs, me.fx, b.v[6]);
+        b.v[7] = __builtin_fmaf(Zs, me.fy, b.v[7]);
+        b.v[8] = __builtin_fmaf(Zs, me.fz, b.v[8]);
+        b.v[9] = __builtin_fmaf(Zs, me.v, b.v[9]);
+    }
+}
+
+// Steps [s_begin, s_begin + SB) of the rotation over one staged tile.
+// MASKED: per-target-slot state: live[p] (slot p pairs with this tile at
+// all), nt[p] (targets of the slot's 64 that exist), diag[p] (the tile is
+// the slot's own: only j > i); cnt sources exist in the tile.
+template <int KIND, bool UNIFORM, bool MASKED, int TPT, int SB>
+__device__ __forceinline__ void sym_steps(const SrcTile &tile, int s_begin, int lane,
+                                          const Part32 *me, const bool *live, const int *nt,
+                                          const bool *diag, int cnt, Acc32s *a, Acc32s &jacc) {
+    const int e0 = lane + s_begin;  // entry of step s_begin
+#pragma unroll
+    for (int u = 0; u < SB; ++u) {
+        const int s = s_begin + u;
+        const int k = (lane + s) & 63;
+        const Part32 sj = load_src(tile, e0 + u);
+        Acc32s b;
+        if (!MASKED) {
+            pair32s<KIND, UNIFORM, false, true>(me[0], sj, true, a[0], b);
+#pragma unroll
+            for (int p = 1; p < TPT; ++p) pair32s<KIND, UNIFORM, false, false>(me[p], sj, true, a[p], b);
+        } else {
+            const bool src_ok = k < cnt;
+#pragma unroll
+            for (int m = 0; m < NCOMP_S; ++m) b.v[m] = 0.0f;
+#pragma unroll
+            for (int p = 0; p < TPT; ++p) {
+                if (!live[p]) continue;
+                const bool valid = src_ok && lane < nt[p] && (!diag[p] || lane + s < 64) &&
+                                   !(diag[p] && s == 0);
+                pair32s<KIND, UNIFORM, true, false>(me[p], sj, valid, a[p], b);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NCOMP_S; ++m) jacc.v[m] = rot_add(jacc.v[m], b.v[m]);
+    }
+}
+
+template <int KIND, int TB, int TPT>
+__global__ __launch_bounds__(TB) void pair32s_kernel(const JobDev *__restrict__ jobs,
+                                                     const Item *__restrict__ items,
+                                                     const int *__restrict__ nitems_p,
+                                                     int *__restrict__ counter) {
+    constexpr int NW = TB / 64;
+    constexpr int TW = TB * TPT;
+    constexpr int SB = PW_SYM_RUN;         // steps per target-side binary32 run
+    constexpr int SBJ = PW_SYM_RUN / TPT;  // steps per source-side run (same term count)
+    __shared__ SrcTile tile;
+    __shared__ double jl[NW][NCOMP_S][64];  // per-source binary64 sums, per wave
+    __shared__ int slot;
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    const int nitems = *nitems_p;
+    for (;;) {
+        const int it = next_item(counter, &slot);
+        if (it >= nitems) break;
+        const Item item = items[it];
+        const JobDev &J = jobs[item.job];
+        const Rec32 *rec = (const Rec32 *)J.rec;
+        const int n = J.n;
+        const int B0 = item.tile * TW;
+        int idx[TPT], wt[TPT], nt[TPT];
+        Part32 me[TPT];
+        double isum[TPT][NCOMP_S];  // target-side sums: 32-term binary32 runs added in binary64
+#pragma unroll
+        for (int p = 0; p < TPT; ++p) {
+            idx[p] = B0 + p * TB + (int)threadIdx.x;
+            wt[p] = (B0 + p * TB) / 64 + w;  // this wave's 64-target tile
+            nt[p] = max(0, min(64, n - wt[p] * 64));
+            const Rec32 r = rec[idx[p] < n ? idx[p] : n - 1];
+            me[p] = Part32{r.x, r.y, r.z, r.ax, r.ay, r.az, r.fx, r.fy, r.fz, r.v, (float)r.h};
+#pragma unroll
+            for (int m = 0; m < NCOMP_S; ++m) isum[p][m] = 0.0;
+        }
+        const bool uniform = J.uniform != 0;
+        const int c0 = max(item.chunk * J.chunk_len, B0);
+        const int c1 = min(item.chunk * J.chunk_len + J.chunk_len, n);
+        const size_t len = (size_t)J.npad - (size_t)B0;  // this block's slot row length
+        double *jrow = J.jpart + jpart_offset((size_t)item.tile, (size_t)J.npad, (size_t)TW);
+        for (int s0 = c0; s0 < c1; s0 += 64) {
+            const int cnt = min(64, c1 - s0);
+            const int st = s0 >> 6;  // source tile
+            __syncthreads();
+            if (threadIdx.x < 64) {  // stage twice; missing sources read as zeros
+                const int k = (int)threadIdx.x;
+                Rec32 r;
+                if (k < cnt) {
+                    r = rec[s0 + k];
+                } else {
+                    r.x = r.y = r.z = r.h = r.ax = r.ay = r.az = 0.0;
+                    r.fx = r.fy = r.fz = r.v = 0.0f;
+                }
+#pragma unroll
+                for (int c = 0; c < 128; c += 64) {
+                    tile.x[k + c] = r.x;
+                    tile.y[k + c] = r.y;
+                    tile.z[k + c] = r.z;
+                    tile.ax[k + c] = r.ax;
+                    tile.ay[k + c] = r.ay;
+                    tile.az[k + c] = r.az;
+                    tile.fx[k + c] = r.fx;
+                    tile.fy[k + c] = r.fy;
+                    tile.fz[k + c] = r.fz;
+                    tile.v[k + c] = r.v;
+                    tile.h[k + c] = (float)r.h;
+                }
+            }
+            __syncthreads();
+            bool live[TPT], diag[TPT], full = cnt == 64, any = false;
+#pragma unroll
+            for (int p = 0; p < TPT; ++p) {
+                live[p] = st >= wt[p] && nt[p] > 0;
+                diag[p] = st == wt[p];
+                full = full && st > wt[p] && nt[p] == 64;
+                any = any || live[p];
+            }
+#pragma unroll
+            for (int m = 0; m < NCOMP_S; ++m) jl[w][m][lane] = 0.0;
+            // other lanes of this wave add into these words next
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            if (any) {
+#pragma unroll 1
+                for (int sb = 0; sb < 64; sb += SB) {
+                    Acc32s a[TPT];
+#pragma unroll
+                    for (int p = 0; p < TPT; ++p)
+#pragma unroll
+                        for (int m = 0; m < NCOMP_S; ++m) a[p].v[m] = 0.0f;
+#pragma unroll 1
+                    for (int sj = sb; sj < sb + SB; sj += SBJ) {
+                        Acc32s jacc;
+#pragma unroll
+                        for (int m = 0; m < NCOMP_S; ++m) jacc.v[m] = 0.0f;
+                        if (full) {
+                            if (uniform)
+                                sym_steps<KIND, true, false, TPT, SBJ>(tile, sj, lane, me, live, nt,
+                                                                       diag, cnt, a, jacc);
+                            else
+                                sym_steps<KIND, false, false, TPT, SBJ>(tile, sj, lane, me, live,
+                                                                        nt, diag, cnt, a, jacc);
+                        } else {
+                            if (uniform)
+                                sym_steps<KIND, true, true, TPT, SBJ>(tile, sj, lane, me, live, nt,
+                                                                      diag, cnt, a, jacc);
+                            else
+                                sym_steps<KIND, false, true, TPT, SBJ>(tile, sj, lane, me, live, nt,
+                                                                       diag, cnt, a, jacc);
+                        }
+                        // end of a source-side run: after step sj + SBJ - 1
+                        // lane l holds the run sums of source
+                        // (l + sj + SBJ - 1) mod 64, one lane per source
+                        const int k = (lane + sj + SBJ - 1) & 63;
+#pragma unroll
+                        for (int m = 0; m < NCOMP_S; ++m) jl[w][m][k] += (double)jacc.v[m];
+                        // the next run's lanes update entries this run's
+                        // lanes wrote
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                    }
+#pragma unroll
+                    for (int p = 0; p < TPT; ++p)
+#pragma unroll
+                        for (int m = 0; m < NCOMP_S; ++m) isum[p][m] += (double)a[p].v[m];
+                }
+            }
+            __syncthreads();
+            // source-side totals of this tile, waves added in order
+            for (int e = threadIdx.x; e < NCOMP_S * 64; e += TB) {
+                const int m = e >> 6, j = e & 63;
+                if (j < cnt) {
+                    double v = jl[0][m][j];
+#pragma unroll
+                    for (int ww = 1; ww < NW; ++ww) v += jl[ww][m][j];
+                    jrow[(size_t)m * len + (size_t)(s0 - B0 + j)] = v;
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < TPT; ++p)
+            if (idx[p] < n) {
+                double *o = J.part + (size_t)item.chunk * NCOMP_S * J.npad + idx[p];
+#pragma unroll
+                for (int m = 0; m < NCOMP_S; ++m) o[(size_t)m * J.npad] = isum[p][m];
+            }
+    }
+}
+
 // --------------------------------------------------------- combine kernel
 // Per particle: add the chunk partials in chunk order (binary64), form the
 // rates
@@ -488,26 +823,60 @@ __global__ __launch_bounds__(256) void combine_kernel(const JobDev *__restrict__
         const int i = c.tile * 256 + (int)threadIdx.x;
         if (i >= J.n) continue;
         const size_t np = (size_t)J.npad;
-        double S[NCOMP];
-#pragma unroll
-        for (int m = 0; m < NCOMP; ++m) S[m] = 0.0;
-        for (int ch = 0; ch < J.nchunk; ++ch) {
-            const double *p = J.part + (size_t)ch * NCOMP * np + i;
-#pragma unroll
-            for (int m = 0; m < NCOMP; ++m) S[m] += p[m * np];
-        }
         double xs[6];
 #pragma unroll
         for (int m = 0; m < 6; ++m) xs[m] = J.xs[m * np + i];
         const double vi = J.vol[i];
-        double k[6];
-        k[0] = J.su * S[0] + J.uinf[0];
-        k[1] = J.su * S[1] + J.uinf[1];
-        k[2] = J.su * S[2] + J.uinf[2];
         const double ax = xs[3], ay = xs[4], az = xs[5];
-        k[3] = J.sq * S[3] + J.sf * S[6] + J.sd * (vi * S[9] - ax * S[12]);
-        k[4] = J.sq * S[4] + J.sf * S[7] + J.sd * (vi * S[10] - ay * S[12]);
-        k[5] = J.sq * S[5] + J.sf * S[8] + J.sd * (vi * S[11] - az * S[12]);
+        double k[6];
+        if (!J.sym) {
+            double S[NCOMP];
+#pragma unroll
+            for (int m = 0; m < NCOMP; ++m) S[m] = 0.0;
+            for (int ch = 0; ch < J.nchunk; ++ch) {
+                const double *p = J.part + (size_t)ch * NCOMP * np + i;
+#pragma unroll
+                for (int m = 0; m < NCOMP; ++m) S[m] += p[m * np];
+            }
+            k[0] = J.su * S[0] + J.uinf[0];
+            k[1] = J.su * S[1] + J.uinf[1];
+            k[2] = J.su * S[2] + J.uinf[2];
+            k[3] = J.sq * S[3] + J.sf * S[6] + J.sd * (vi * S[9] - ax * S[12]);
+            k[4] = J.sq * S[4] + J.sf * S[7] + J.sd * (vi * S[10] - ay * S[12]);
+            k[5] = J.sq * S[5] + J.sf * S[8] + J.sd * (vi * S[11] - az * S[12]);
+        } else {
+            // target side: the chunks from the one holding this particle's
+            // block start; source side: the blocks up to this particle's own
+            const size_t tw = (size_t)J.tw;
+            const int blk = i / J.tw;
+            const int cfirst = (int)(((size_t)blk * tw) / (size_t)J.chunk_len);
+            double S[NCOMP_S], T[NCOMP_S];
+#pragma unroll
+            for (int m = 0; m < NCOMP_S; ++m) S[m] = T[m] = 0.0;
+            for (int ch = cfirst; ch < J.nchunk; ++ch) {
+                const double *p = J.part + (size_t)ch * NCOMP_S * np + i;
+#pragma unroll
+                for (int m = 0; m < NCOMP_S; ++m) S[m] += p[m * np];
+            }
+            for (int b = 0; b <= blk; ++b) {
+                const size_t len = np - (size_t)b * tw;
+                const double *q = J.jpart + jpart_offset((size_t)b, np, tw) + (i - (size_t)b * tw);
+#pragma unroll
+                for (int m = 0; m < NCOMP_S; ++m) T[m] += q[m * len];
+            }
+            double U[NCOMP_S];
+#pragma unroll
+            for (int m = 0; m < NCOMP_S; ++m) U[m] = S[m] + T[m];
+            U[3] = S[3] - T[3];
+            U[4] = S[4] - T[4];
+            U[5] = S[5] - T[5];
+            k[0] = J.su * U[0] + J.uinf[0];
+            k[1] = J.su * U[1] + J.uinf[1];
+            k[2] = J.su * U[2] + J.uinf[2];
+            k[3] = J.sq * U[3] + J.sd * (vi * U[6] - ax * U[9]);
+            k[4] = J.sq * U[4] + J.sd * (vi * U[7] - ay * U[9]);
+            k[5] = J.sq * U[5] + J.sd * (vi * U[8] - az * U[9]);
+        }
         bool finite = true;
 #pragma unroll
         for (int m = 0; m < 6; ++m) {

@@ -101,13 +101,21 @@ static void fill_records(void *dst, int precision, size_t m, const double *pos, 
     }
 }
 
-size_t job_footprint(size_t n, int precision, int TB) {
+size_t jpart_count(size_t n, size_t npad, size_t tw) {
+    const size_t nblk = (n + tw - 1) / tw;
+    return jpart_offset(nblk, npad, tw);
+}
+
+size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym32) {
     size_t npad = round_up(std::max<size_t>(n, 1), 256);
     size_t L = chunk_for(n, TB);
     size_t nchunk = (n + L - 1) / L;
-    return npad * (rec_bytes(precision) + 5 * 6 * sizeof(double) + 2 * sizeof(double) +
-                   nchunk * NCOMP * sizeof(double)) +
-           256;
+    size_t b = npad * (rec_bytes(precision) + 5 * 6 * sizeof(double) + 2 * sizeof(double) +
+                       nchunk * NCOMP * sizeof(double)) +
+               256;
+    if (precision == PREC_FP32C && sym32)
+        b += jpart_count(n, npad, (size_t)TB * TPT) * sizeof(double);
+    return b;
 }
 
 JobDev Job::dev() const {
@@ -130,6 +138,9 @@ JobDev Job::dev() const {
     d.chunk_len = chunk_len;
     d.prec = par.precision;
     d.uniform = uniform ? 1 : 0;
+    d.sym = sym ? 1 : 0;
+    d.tw = tw;
+    d.jpart = jpart.as<double>();
     d.dt = par.dt;
     d.nu = par.nu;
     for (int c = 0; c < 3; ++c) d.uinf[c] = par.uinf[c];
@@ -168,6 +179,22 @@ typedef void (*PairFn)(const JobDev *, const Item *, const int *, int *);
     X(KIND, 256, 1, 2) X(KIND, 256, 1, 4) X(KIND, 256, 2, 1) X(KIND, 256, 2, 2)                \
     X(KIND, 256, 2, 4)
 
+// Symmetric binary32 pass: every (block, targets per thread) for both kernels.
+static PairFn pair_fn_sym(int kind, int tb, int tpt) {
+#define PW_CASES(K, B, T) \
+    if (tb == B && tpt == T) return pair32s_kernel<K, B, T>;
+    if (kind == KIND_HIGHORDER) {
+        PW_CASES(KIND_HIGHORDER, 64, 1) PW_CASES(KIND_HIGHORDER, 64, 2)
+        PW_CASES(KIND_HIGHORDER, 128, 1) PW_CASES(KIND_HIGHORDER, 128, 2)
+        PW_CASES(KIND_HIGHORDER, 256, 1) PW_CASES(KIND_HIGHORDER, 256, 2)
+    } else {
+        PW_CASES(KIND_GAUSSIAN, 64, 1) PW_CASES(KIND_GAUSSIAN, 128, 1)
+        PW_CASES(KIND_GAUSSIAN, 256, 1)
+    }
+#undef PW_CASES
+    return nullptr;
+}
+
 static PairFn pair_fn(int prec, int kind, int tb, int tpt, int unr) {
 #define PW_CASE64(K, B, T, U) \
     if (tb == B && tpt == T && unr == U) return pair64_kernel<K, B, T, U>;
@@ -195,17 +222,19 @@ static PairFn pair_fn(int prec, int kind, int tb, int tpt, int unr) {
     return nullptr;
 }
 
-bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32) {
-    return pair_fn(PREC_FP64, kind, tb, tpt, unr64) && pair_fn(PREC_FP32C, kind, tb, tpt, unr32);
+bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32, int sym32) {
+    return pair_fn(PREC_FP64, kind, tb, tpt, unr64) &&
+           (sym32 ? pair_fn_sym(kind, tb, tpt) != nullptr
+                  : pair_fn(PREC_FP32C, kind, tb, tpt, unr32) != nullptr);
 }
 
 Engine::Engine(int device, int kind, const Variant &v)
     : device_(device), kind_(kind), tb_(v.block), tpt_(v.tpt), unr64_(v.unroll64),
-      unr32_(v.unroll32) {
+      unr32_(v.unroll32), sym32_(v.sym32 ? 1 : 0) {
     if (tb_ != 64 && tb_ != 128 && tb_ != 256) throw Error(PW_EINVAL, "block must be 64, 128 or 256");
     if (kind_ != KIND_HIGHORDER && kind_ != KIND_GAUSSIAN) throw Error(PW_EINVAL, "unknown kernel");
     fn64_ = pair_fn(PREC_FP64, kind_, tb_, tpt_, unr64_);
-    fn32_ = pair_fn(PREC_FP32C, kind_, tb_, tpt_, unr32_);
+    fn32_ = sym32_ ? pair_fn_sym(kind_, tb_, tpt_) : pair_fn(PREC_FP32C, kind_, tb_, tpt_, unr32_);
     if (!fn64_ || !fn32_) throw Error(PW_EINVAL, "no compiled kernel variant for these parameters");
     int ndev = 0;
     PW_HIP(hipGetDeviceCount(&ndev));
@@ -280,7 +309,11 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     j->part.reset((size_t)j->nchunk * NCOMP * np * sizeof(double));
     j->vol.reset(2 * np * sizeof(double));  // volume | radius
     j->flag.reset(256);
-    j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + j->flag.bytes;
+    j->sym = par.precision == PREC_FP32C && sym32_;
+    j->tw = tb_ * tpt_;
+    alloc_jpart(j.get());
+    j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + j->flag.bytes +
+               j->jpart.bytes;
     // host-side staging of records, state and constants
     size_t need = std::max(np * rec_bytes(par.precision), 6 * np * sizeof(double));
     stage_.ensure(std::max(need, 2 * np * sizeof(double)));
@@ -386,8 +419,18 @@ void Engine::append(Job *j, size_t m, const double *pos, const double *vort, con
         j->part_chunks = std::max(j->nchunk, 16);
         j->part.reset((size_t)j->part_chunks * NCOMP * np * sizeof(double));
     }
-    j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + j->flag.bytes;
+    alloc_jpart(j);
+    j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + j->flag.bytes +
+               j->jpart.bytes;
     plan_dirty_ = true;
+}
+
+// Source-side slots of a symmetric job, sized for its current n and stride;
+// they only grow.  Scratch like the partial sums: rewritten by every pass.
+void Engine::alloc_jpart(Job *j) {
+    if (!j->sym) return;
+    const size_t need = jpart_count(j->n, j->npad, (size_t)j->tw) * sizeof(double);
+    if (need > j->jpart.bytes) j->jpart.reset(need);
 }
 
 void Engine::remove_job(int64_t id) {
@@ -419,8 +462,21 @@ void Engine::build_plan() {
         const Job &j = *jobs_[k];
         jd.push_back(j.dev());
         auto &dst = j.par.precision == PREC_FP64 ? it64 : it32;
-        for (int c = 0; c < j.nchunk; ++c)
-            for (int t = 0; t < j.ntile; ++t) dst.push_back(Item{(int)k, t, c, j.chunk_len});
+        if (!j.sym) {
+            for (int c = 0; c < j.nchunk; ++c)
+                for (int t = 0; t < j.ntile; ++t) dst.push_back(Item{(int)k, t, c, j.chunk_len});
+        } else {
+            // block t pairs with the sources from its own start on: chunks
+            // from the one holding that start; cost = sources visited
+            for (int t = 0; t < j.ntile; ++t) {
+                const size_t b0 = (size_t)t * j.tw;
+                for (int c = (int)(b0 / j.chunk_len); c < j.nchunk; ++c) {
+                    const size_t lo = std::max(b0, (size_t)c * j.chunk_len);
+                    const size_t hi = std::min(j.n, (size_t)(c + 1) * j.chunk_len);
+                    dst.push_back(Item{(int)k, t, c, (int)(hi - lo)});
+                }
+            }
+        }
         for (int t = 0; t < (int)((j.n + 255) / 256); ++t) ci.push_back(CItem{(int)k, t});
     }
     // larger items first (their cost is tile x chunk length): shorter tail

@@ -61,8 +61,12 @@ struct JobParams {
 // so a job's sums are formed the same way alone or in any batch.
 size_t chunk_for(size_t n, int TB);
 
-// Bytes a resident job occupies on the device.
-size_t job_footprint(size_t n, int precision, int TB);
+// Bytes a resident job occupies on the device (TB x TPT targets per block;
+// sym32: a binary32 job uses the symmetric pass and its source-side slots).
+size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym32);
+// Binary64 sums in a symmetric job's source-side slots (block rows of
+// NCOMP_S x (npad - b tw) for every block b).
+size_t jpart_count(size_t n, size_t npad, size_t tw);
 
 struct Diagnostics {
     double omega[3], impulse[3], angular[3];
@@ -79,6 +83,9 @@ struct Job {
     bool uniform = false;
     double sigma = 0.0;
     DevBuf rec, soa, part, vol, flag;  // soa holds x0, xs, k1, k23, rate
+    DevBuf jpart;               // source-side slots of the symmetric pass
+    bool sym = false;           // binary32 job on the symmetric pass
+    int tw = 0;                 // targets per block of the pass
     size_t bytes = 0;
     size_t chunk_override = 0;  // fixed chunk length (0: policy)
     int part_chunks = 0;        // chunks the partial-sum buffer holds
@@ -91,8 +98,9 @@ struct Job {
 // unroll of the binary64 and binary32 pair kernels.
 struct Variant {
     int block = 128, tpt = 1, unroll64 = 2, unroll32 = 4;
+    int sym32 = 1;  // binary32 jobs: 1 symmetric pass, 0 one-sided pass
 };
-bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32);
+bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32, int sym32 = 1);
 
 class Engine {
   public:
@@ -104,7 +112,7 @@ class Engine {
     int device() const { return device_; }
     int kind() const { return kind_; }
     int block() const { return tb_; }
-    Variant variant() const { return Variant{tb_, tpt_, unr64_, unr32_}; }
+    Variant variant() const { return Variant{tb_, tpt_, unr64_, unr32_, sym32_}; }
     hipStream_t stream() const { return stream_; }
 
     // Create a job from N x 3 positions/vorticities and N radius/volume.
@@ -155,7 +163,8 @@ class Engine {
     void capture_graph();
     double pairs_per_rhs() const;
 
-    int device_, kind_, tb_, tpt_, unr64_, unr32_;
+    int device_, kind_, tb_, tpt_, unr64_, unr32_, sym32_;
+    void alloc_jpart(Job *j);
     void (*fn64_)(const JobDev *, const Item *, const int *, int *) = nullptr;
     void (*fn32_)(const JobDev *, const Item *, const int *, int *) = nullptr;
     hipStream_t stream_ = nullptr;

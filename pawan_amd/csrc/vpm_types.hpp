@@ -14,6 +14,11 @@ constexpr int PREC_FP32C = 1;
 //   0-2 sum q (a_j x d)   3-5 sum q (a_i x a_j)   6-8 sum F (d.(a_i x a_j)) d
 //   9-11 sum k a_j        12 sum k v_j
 constexpr int NCOMP = 13;
+// Symmetric binary32 pass (each unordered pair once): per (particle, slot)
+//   0-2 velocity   3-5 stretching q e + F (d.e) d   6-8 sum k a   9 sum k v
+// for the target side (slots per source chunk) and for the source side
+// (slots per target block; the stretching there enters with a minus sign).
+constexpr int NCOMP_S = 10;
 
 // Combine-kernel modes
 constexpr int MODE_SOLVE = 0;
@@ -83,13 +88,24 @@ struct JobDev {
     const double *vol;  // [npad] volumes
     int *flag;      // set to 1 when a rate or state is non-finite
     int n, npad, nchunk, chunk_len;
-    int prec, uniform, pad0, pad1;
+    int prec, uniform;
+    int sym;        // binary32 symmetric pass: part holds NCOMP_S target-side
+                    // components, jpart the source-side ones
+    int tw;         // targets per block of the symmetric pass
+    double *jpart;  // source-side slots: block b holds [NCOMP_S][npad - b tw]
+                    // sums for particles b tw .. npad-1, at jpart_offset(b)
     double inv_s2;  // non-uniform core: unused; uniform: 1 (positions are prescaled)
     double rec_scale;  // records hold positions x rec_scale (1/sigma for a uniform core)
     double su, sq, sf, sd;  // epilogue scales (see combine_kernel)
     double dt, nu;
     double uinf[3];
 };
+
+// Start of block b's source-side slot row (rows shrink by tw per block:
+// block b only reaches particles from b tw on).
+__host__ __device__ inline size_t jpart_offset(size_t b, size_t npad, size_t tw) {
+    return (size_t)NCOMP_S * (b * npad - tw * (b * (b - 1) / 2));
+}
 
 struct Item {
     int job, tile, chunk, pad;

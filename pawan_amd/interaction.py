@@ -70,7 +70,8 @@ class Interaction:
 
     def __init__(self, *wakes, nu: float = 2.5e-3, kernel: int = _lib.KERNEL_HIGHORDER,
                  precision: int = _lib.FP64, uinf=(0.0, 0.0, 0.0), device: int = 0,
-                 block: int = 0, chunk: int = 0, targets_per_thread: int = 0, unroll: int = 0):
+                 block: int = 0, chunk: int = 0, targets_per_thread: int = 0, unroll: int = 0,
+                 scheme: int = _lib.SCHEME_DEFAULT):
         L = lib()
         cfg = _lib.PwConfig()
         L.pw_config_default(C.byref(cfg))
@@ -78,6 +79,7 @@ class Interaction:
         cfg.uinf[:] = list(uinf)
         cfg.block, cfg.chunk = block, chunk
         cfg.targets_per_thread, cfg.unroll = targets_per_thread, unroll
+        cfg.scheme = scheme
         h = C.c_void_p()
         rc = L.pw_create(C.byref(h), C.byref(cfg))
         if rc != _lib.PW_OK:
@@ -263,9 +265,9 @@ class Batch:
     """Many independent simulations stepped in one persistent launch."""
 
     def __init__(self, device: int = 0, kernel: int = _lib.KERNEL_HIGHORDER, block: int = 0,
-                 targets_per_thread: int = 0, unroll: int = 0):
+                 targets_per_thread: int = 0, unroll: int = 0, scheme: int = _lib.SCHEME_DEFAULT):
         h = C.c_void_p()
-        v = _lib.PwVariant(block, targets_per_thread, unroll)
+        v = _lib.PwVariant(block, targets_per_thread, unroll, scheme)
         rc = lib().pw_batch_create(C.byref(h), device, kernel, C.byref(v))
         if rc != _lib.PW_OK:
             raise _lib.PawanError(rc, lib().pw_batch_last_error(None).decode())
@@ -338,9 +340,9 @@ class Server:
 
     def __init__(self, hbm_budget_bytes: int, device: int = 0,
                  kernel: int = _lib.KERNEL_HIGHORDER, block: int = 0, targets_per_thread: int = 0,
-                 unroll: int = 0):
+                 unroll: int = 0, scheme: int = _lib.SCHEME_DEFAULT):
         h = C.c_void_p()
-        v = _lib.PwVariant(block, targets_per_thread, unroll)
+        v = _lib.PwVariant(block, targets_per_thread, unroll, scheme)
         rc = lib().pw_server_create(C.byref(h), device, kernel, hbm_budget_bytes, C.byref(v))
         if rc != _lib.PW_OK:
             raise _lib.PawanError(rc, lib().pw_server_last_error(None).decode())
@@ -414,7 +416,8 @@ def autotune(w: dict, precision: int = _lib.FP64, kernel: int = _lib.KERNEL_HIGH
 
     def d(e):
         return {"block": e.variant.block, "targets_per_thread": e.variant.targets_per_thread,
-                "unroll": e.variant.unroll, "pair_ms": e.pair_ms, "pairs_per_s": e.pairs_per_s}
+                "unroll": e.variant.unroll, "scheme": e.variant.scheme, "pair_ms": e.pair_ms,
+                "pairs_per_s": e.pairs_per_s}
 
     if cache:
         lib().pw_tune_save(str(cache).encode(), kernel, precision, n, C.byref(best))

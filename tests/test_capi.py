@@ -49,7 +49,14 @@ def test_host_only_entry_points():
             assert L % b == 0 and (n + L - 1) // L <= 16
     assert lib.pw_job_footprint(262144, 0) > 262144 * 64
     # binary32 records carry binary64 strengths too: 80 B instead of 64 B
-    assert lib.pw_job_footprint(262144, 1) - lib.pw_job_footprint(262144, 0) == 16 * 262144
+    # binary32: 16 more bytes per record, plus the symmetric pass's source-side
+    # slots, NCOMP_S = 10 binary64 sums per (particle, block of 256 x 2 targets) for
+    # the blocks at or before the particle's own
+    tw, npad = 512, 262144
+    nblk = npad // tw
+    slots = 10 * (nblk * npad - tw * nblk * (nblk - 1) // 2)
+    assert (lib.pw_job_footprint(262144, 1) - lib.pw_job_footprint(262144, 0)
+            == 16 * 262144 + 8 * slots)
 
 
 def test_null_arguments_rejected():
@@ -91,13 +98,13 @@ def test_tune_cache_roundtrip(tmp_path):
     assert lib.pw_tune_lookup(path, 0, 0, 262144, C.byref(v)) == _lib.PW_ENOTFOUND
     e = _lib.PwTuneEntry(_lib.PwVariant(256, 2, 4), 123.0, 5e11)
     assert lib.pw_tune_save(path, 0, 0, 262144, C.byref(e)) == 0
-    e2 = _lib.PwTuneEntry(_lib.PwVariant(64, 1, 1), 1.0, 1e9)
+    e2 = _lib.PwTuneEntry(_lib.PwVariant(64, 2, 0, _lib.SCHEME_SYMMETRIC), 1.0, 1e9)
     assert lib.pw_tune_save(path, 0, 1, 2048, C.byref(e2)) == 0
     e3 = _lib.PwTuneEntry(_lib.PwVariant(128, 1, 2), 120.0, 5e11)
     assert lib.pw_tune_save(path, 0, 0, 300000, C.byref(e3)) == 0  # same log2 bucket: replaces
     assert lib.pw_tune_lookup(path, 0, 0, 262144, C.byref(v)) == 0
     assert (v.block, v.targets_per_thread, v.unroll) == (128, 1, 2)
     assert lib.pw_tune_lookup(path, 0, 1, 3000, C.byref(v)) == 0
-    assert (v.block, v.targets_per_thread, v.unroll) == (64, 1, 1)
+    assert (v.block, v.targets_per_thread, v.unroll, v.scheme) == (64, 2, 0, _lib.SCHEME_SYMMETRIC)
     assert lib.pw_tune_lookup(path, 1, 0, 262144, C.byref(v)) == _lib.PW_ENOTFOUND
     assert len(open(path.decode()).read().splitlines()) == 2

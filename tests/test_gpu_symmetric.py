@@ -1,0 +1,108 @@
+"""The symmetric binary32 pair pass (each unordered pair once, both particles
+updated, source-side sums in per-block slots) against the CPU oracle, on the
+shapes where its bookkeeping can go wrong: particle counts around the 64-wide
+tiles and the target blocks, a block straddling a chunk boundary, several
+wakes, non-uniform cores, the Gaussian kernel, a freestream; and against the
+one-sided binary32 pass.  Tolerance: 1e-6 relative RMS per rate field
+(velocity, vorticity rate), as for every binary32 result."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import pawan_amd as pa
+from pawan_amd import generators as gen
+from helpers import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 1e-6
+
+
+def _random_blob(n, seed, core=0.05, spread=0.5):
+    u = gen.uniform(seed, 7 * n).reshape(n, 7)
+    pos = spread * (u[:, :3] - 0.5)
+    vort = 0.01 * (u[:, 3:6] - 0.5)
+    return {"pos": np.ascontiguousarray(pos), "vort": np.ascontiguousarray(vort),
+            "radius": np.full(n, core), "volume": np.full(n, 4.0 / 3.0 * np.pi * core ** 3),
+            "n": n}
+
+
+def _rates(w, nu, precision, scheme=pa.SCHEME_DEFAULT, wake_n=None, **kw):
+    wake_n = wake_n or [w["pos"].shape[0]]
+    offs = np.cumsum([0] + list(wake_n))
+    wakes = [pa.Wake(w["pos"][a:b], w["vort"][a:b], w["radius"][a:b], w["volume"][a:b])
+             for a, b in zip(offs[:-1], offs[1:])]
+    S = pa.Interaction(*wakes, nu=nu, precision=precision, scheme=scheme, **kw)
+    S.solve()
+    r = S.getRates()
+    S.close()
+    return orc.unpack_state(r, wake_n)
+
+
+@pytest.mark.parametrize("n", [2, 3, 63, 64, 65, 127, 200, 511, 513, 1000])
+@pytest.mark.parametrize("block,tpt", [(64, 1), (128, 2), (256, 1), (256, 2)])
+def test_symmetric_tile_edges(n, block, tpt):
+    """Random blobs: every pair counted once.  Random strengths make the sums
+    cancel far more than any vortex workload does, so the binary32 rounding
+    shows at ~1e-6 here; the bound is 1e-5, far below what one missing or
+    doubled pair would cost (~1/n of the RMS rate)."""
+    w = _random_blob(n, seed=1000 + n)
+    print(f"case symmetric n={n} block={block} tpt={tpt}", flush=True)
+    nu = 1e-3
+    vel, dv = _rates(w, nu, pa.FP32C, block=block, targets_per_thread=tpt)
+    vo, do = _rates(w, nu, pa.FP32C, pa.SCHEME_ONESIDED, block=block, targets_per_thread=tpt)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL)
+    ev, ed = rel_rms(vel, rv), rel_rms(dv, rd)
+    print(f"  symmetric vel {ev:.2e} dvort {ed:.2e}; one-sided vel {rel_rms(vo, rv):.2e} "
+          f"dvort {rel_rms(do, rd):.2e}", flush=True)
+    assert ev < 1e-5 and ed < 1e-5, (ev, ed)
+
+
+def test_symmetric_single_particle():
+    w = _random_blob(1, seed=7)
+    vel, dv = _rates(w, 1e-3, pa.FP32C, uinf=(0.25, 0.0, -0.5))
+    assert np.array_equal(vel, np.array([[0.25, 0.0, -0.5]])) and np.all(dv == 0.0)
+
+
+@pytest.mark.parametrize("block,tpt,chunk", [(128, 2, 384), (64, 2, 192), (256, 2, 768)])
+def test_symmetric_block_straddles_chunks(block, tpt, chunk):
+    """Chunks that are not multiples of the target block: a block's own
+    sources start in the middle of a chunk and run into the next one."""
+    w = gen.vortex_ring(60, 30, R=1.0, core=0.1, seed=21)  # 1,800 particles
+    print(f"case straddle block={block} tpt={tpt} chunk={chunk}", flush=True)
+    nu = 1e-3
+    vel, dv = _rates(w, nu, pa.FP32C, block=block, targets_per_thread=tpt, chunk=chunk)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.OMP_CLEAN)
+    ev, ed = rel_rms(vel, rv), rel_rms(dv, rd)
+    print(f"  vel {ev:.2e} dvort {ed:.2e}", flush=True)
+    assert ev < TOL32 and ed < TOL32
+
+
+def test_symmetric_wakes_nonuniform_gaussian_freestream():
+    a = gen.vortex_ring(40, 16, R=1.0, core=0.1, seed=31)
+    b = gen.vortex_ring(30, 16, R=0.7, core=0.08, seed=32, center=(0.1, 0, 0.3))
+    w = {k: np.concatenate([a[k], b[k]]) for k in ("pos", "vort", "radius", "volume")}
+    w["radius"] = w["radius"] * (1.0 + 0.3 * gen.uniform(33, w["pos"].shape[0]))
+    wake_n = [a["n"], b["n"]]
+    nu, uinf = 2e-3, (0.0, 0.05, -0.1)
+    for kernel, kind in ((pa.KERNEL_HIGHORDER, orc.HIGHORDER), (pa.KERNEL_GAUSSIAN, orc.GAUSSIAN)):
+        print(f"case wakes nonuniform kernel={kernel}", flush=True)
+        vel, dv = _rates(w, nu, pa.FP32C, wake_n=wake_n, kernel=kernel, uinf=uinf)
+        rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL,
+                           kind=kind, wake_n=wake_n, uinf=uinf)
+        ev, ed = rel_rms(vel, rv), rel_rms(dv, rd)
+        print(f"  vel {ev:.2e} dvort {ed:.2e}", flush=True)
+        assert ev < TOL32 and ed < TOL32
+
+
+def test_symmetric_vs_onesided_rotor():
+    w = gen.rotor_wake(32)  # 8,192 particles
+    print(f"case rotor N={w['n']}", flush=True)
+    nu = 1e-4
+    v64, d64 = _rates(w, nu, pa.FP64)
+    vs, ds = _rates(w, nu, pa.FP32C, pa.SCHEME_SYMMETRIC)
+    vo, do = _rates(w, nu, pa.FP32C, pa.SCHEME_ONESIDED)
+    for name, v, d in (("symmetric", vs, ds), ("one-sided", vo, do)):
+        ev, ed = rel_rms(v, v64), rel_rms(d, d64)
+        print(f"  {name}: vel {ev:.2e} dvort {ed:.2e}", flush=True)
+        assert ev < TOL32 and ed < TOL32
