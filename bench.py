@@ -35,10 +35,11 @@ sys.path.insert(0, ROOT)
 FLOP_PER_PAIR = 95.0          # algorithmic FLOP per ordered pair (one-sided form)
 FP64_VECTOR_PEAK = 78.6e12    # MI355X FP64 vector FLOP/s
 FP32_VECTOR_PEAK = 157.3e12   # MI355X FP32 vector FLOP/s (published, packed)
-# HBM bytes per binary64 pair-kernel launch at N = 262,144, from rocprofv3
-# FETCH_SIZE + WRITE_SIZE passes (KiB per dispatch, write factor 1.00 from a
-# copy kernel of known size); profiles/pair_kernels_rotor_wake_262144.md
-PAIR64_TRAFFIC_262144 = (229_000 + 426_528) * 1024.0
+# HBM bytes per binary64 pair-kernel launch at N = 262,144 (256 x 2 variant),
+# from rocprofv3 FETCH_SIZE + WRITE_SIZE passes in KiB per dispatch, scaled by
+# what the same counters read for a cold 2 GiB copy of known size (fetch x2,
+# write x1); profiles/pair_kernels_rotor_wake_262144.md
+PAIR64_TRAFFIC_262144 = (2 * 212_106 + 426_256) * 1024.0
 
 
 def log(*a):
@@ -54,6 +55,14 @@ def log(*a):
 # costs the same for any dt.
 STEP_DT = 1e-3
 LONG_DT = 1e-5
+
+# The variants the autotuner picks for the N = 262,144 rotor wake on MI355X
+# (used with --no-tune, e.g. under a profiler): binary64 one-sided 256 x 2
+# targets, unroll 2; compensated binary32 symmetric 256 x 1.
+MEASURED_BEST = {
+    "fp64": {"block": 256, "targets_per_thread": 2, "unroll": 2, "scheme": 1},
+    "fp32c": {"block": 256, "targets_per_thread": 1, "unroll": 0, "scheme": 2},
+}
 
 
 def run_path(pa, w, precision, steps, warmup, variant=None):
@@ -187,6 +196,26 @@ def serving_run(pa, gen, duration_s=20.0, rate_hz=4.0, seed=1, budget=200 << 30,
         "steps": st["steps"], "n_range": [n_lo, n_hi], "steps_range": [s_lo, s_hi],
         "still_running": st["running"] + st["queued"],
     }
+
+
+def copy_calibration(mib):
+    """Two device-to-device copies of mib MiB between fresh buffers larger
+    than the caches: under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, the known
+    byte count of this kernel gives the scale of the counters (torch owns the
+    memory and the copy kernel)."""
+    import torch
+    n = mib << 20
+    out = []
+    for _ in range(2):
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty(n, dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        out.append(time.perf_counter() - t0)
+        del a, b
+    return {"copy_mib": mib, "bytes_read": n, "bytes_written": n, "seconds": out}
 
 
 # ------------------------------------------------------------------ workloads
@@ -361,7 +390,11 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
-    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner; default variant")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device (twice, fresh buffers): the "
+                         "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner; run the variants it picks on MI355X at N=262,144")
     ap.add_argument("--workload", default="rotor262k",
                     choices=["rotor262k", "ring1", "leapfrog", "rotor_growth", "batch", "serve"])
     ap.add_argument("--shed-steps", type=int, default=1024)
@@ -373,6 +406,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        print(json.dumps(copy_calibration(args.copy_calibration)), flush=True)
+        return
 
     import pawan_amd as pa
     from pawan_amd import generators as gen
@@ -385,7 +421,7 @@ def main():
 
     w = gen.config_rotor_wake(args.n_shed)
     n = w["n"]
-    tuned = {}
+    tuned = dict(MEASURED_BEST) if args.no_tune else {}
     if not args.no_tune:
         for name, prec in (("fp64", pa.FP64), ("fp32c", pa.FP32C)):
             if name == "fp32c" and args.no_fp32c:
@@ -418,10 +454,12 @@ def main():
                    "step": "one RK4 step = 4 all-pairs RHS", "dt": STEP_DT, "pairs_per_step": 4.0 * n * (n - 1)},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": FP64_VECTOR_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / FP64_VECTOR_PEAK,
-                     "traffic": PAIR64_TRAFFIC_262144 if n == 262144 else None,
+                     "traffic": (PAIR64_TRAFFIC_262144 if n == 262144 and
+                                 tuned.get("fp64", {}).get("block") == 256 and
+                                 tuned.get("fp64", {}).get("targets_per_thread") == 2 else None),
                      "kernel": "pair64_kernel", "flop_per_pair": FLOP_PER_PAIR,
                      "launch_ms": r64["pair_kernel_ms_mean"]},
-        "tuned_variants": tuned,
+        "variants": tuned, "autotuned": not args.no_tune,
     }
     if not args.no_fp32c:
         log("fp32c path")
