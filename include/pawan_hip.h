@@ -190,7 +190,8 @@ typedef struct pw_job_spec {
     int precision;        /* PW_FP64 / PW_FP32C                            */
 } pw_job_spec;
 
-/* variant may be NULL (block 128, one target per thread, default unroll) */
+/* variant may be NULL; zero fields take the defaults: block 256, two targets
+ * per thread, default unroll, symmetric binary32 pass */
 int pw_batch_create(pw_batch **out, int device, int kernel, const pw_variant *variant);
 int pw_batch_destroy(pw_batch *b);
 const char *pw_batch_last_error(const pw_batch *b);

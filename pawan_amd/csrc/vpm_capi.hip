@@ -433,13 +433,18 @@ static JobParams params_of(const pw_job_spec *s) {
 
 extern "C" {
 
+// Batch and server engines: 256 threads x 2 targets unless given (the best
+// binary64 variant at large N, and half the binary32 symmetric pass's slot
+// memory of one target per thread).
 static pw::Variant variant_of(const pw_variant *v, int precision_hint) {
     pw::Variant r;
+    r.block = 256;
+    r.tpt = 2;
     if (!v) return r;
     if (v->scheme < PW_SCHEME_DEFAULT || v->scheme > PW_SCHEME_SYMMETRIC)
         throw Error(PW_EINVAL, "unknown pair scheme");
-    r.block = v->block ? v->block : 128;
-    r.tpt = v->targets_per_thread ? v->targets_per_thread : 1;
+    r.block = v->block ? v->block : 256;
+    r.tpt = v->targets_per_thread ? v->targets_per_thread : 2;
     r.sym32 = v->scheme != PW_SCHEME_ONESIDED;
     if (v->unroll) {
         if (precision_hint == PW_FP32C)

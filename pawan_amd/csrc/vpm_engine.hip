@@ -114,7 +114,7 @@ size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym32) {
                        nchunk * NCOMP * sizeof(double)) +
                256;
     if (precision == PREC_FP32C && sym32)
-        b += jpart_count(n, npad, (size_t)TB * TPT) * sizeof(double);
+        b += jpart_count(npad, npad, (size_t)TB * TPT) * sizeof(double);
     return b;
 }
 
@@ -425,11 +425,13 @@ void Engine::append(Job *j, size_t m, const double *pos, const double *vort, con
     plan_dirty_ = true;
 }
 
-// Source-side slots of a symmetric job, sized for its current n and stride;
-// they only grow.  Scratch like the partial sums: rewritten by every pass.
+// Source-side slots of a symmetric job, sized for its padded capacity (npad
+// particles), so that particle emission reallocates them only when the
+// stride grows (geometrically), not on every append.  Scratch like the
+// partial sums: rewritten by every pass.
 void Engine::alloc_jpart(Job *j) {
     if (!j->sym) return;
-    const size_t need = jpart_count(j->n, j->npad, (size_t)j->tw) * sizeof(double);
+    const size_t need = jpart_count(j->npad, j->npad, (size_t)j->tw) * sizeof(double);
     if (need > j->jpart.bytes) j->jpart.reset(need);
 }
 

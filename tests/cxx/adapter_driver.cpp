@@ -82,6 +82,20 @@ int main(int argc, char **argv) {
         double ev = relerr(gv, vel), ed = relerr(gd, dv);
         std::printf("driver: solve() vs oracle: velocity %.3e vorticity rate %.3e\n", ev, ed);
         bad |= !(ev < 1e-10 && ed < 1e-10);
+        {  // the compensated binary32 path (symmetric pass) through the same adapter
+            pawan_gpu::options o32 = o;
+            o32.precision = PW_FP32C;
+            gpu_interaction S32(W1, W2, o32);
+            S32.solve();
+            std::vector<double> sv(W1->_velocity->data, W1->_velocity->data + 3 * n1),
+                sd(W1->_retvorcity->data, W1->_retvorcity->data + 3 * n1);
+            sv.insert(sv.end(), W2->_velocity->data, W2->_velocity->data + 3 * n2);
+            sd.insert(sd.end(), W2->_retvorcity->data, W2->_retvorcity->data + 3 * n2);
+            double e32v = relerr(sv, vel), e32d = relerr(sd, dv);
+            std::printf("driver: binary32 solve() vs oracle: velocity %.3e vorticity rate %.3e\n",
+                        e32v, e32d);
+            bad |= !(e32v < 1e-5 && e32d < 1e-5);
+        }
         // resolve(): each wake's _vorticityfield
         std::vector<double> fld(3 * n);
         orc_influence(ORC_KERNEL_HIGHORDER, 2, wn, pos.data(), vort.data(), rad.data(), fld.data());
