@@ -55,14 +55,15 @@ extern "C" {
 #define PW_FP32C 1 /* binary32 pair terms, compensated per-target sums,
                       binary64 cross-tile reduction, state and diagnostics    */
 
-/* how the binary32 pass visits pairs (binary64 always uses one-sided) */
-#define PW_SCHEME_DEFAULT 0   /* symmetric                                    */
+/* how the pair pass visits pairs */
+#define PW_SCHEME_DEFAULT 0   /* binary32 symmetric, binary64 one-sided (the
+                                 plain method: a thread per target)            */
 #define PW_SCHEME_ONESIDED 1  /* every ordered pair, per target (no source-side
-                                 sums)                                         */
+                                 sums), both precisions                        */
 #define PW_SCHEME_SYMMETRIC 2 /* every unordered pair once, both particles
                                  updated, as the reference's INTERACT does
                                  (interaction_utils.h:429-481); deterministic
-                                 per-block source-side slots                   */
+                                 per-block source-side slots; both precisions  */
 
 typedef struct pw_config {
     int device;       /* HIP device ordinal                                      */
@@ -74,7 +75,7 @@ typedef struct pw_config {
     int chunk;        /* sources per work item (multiple of block); 0 = policy   */
     int targets_per_thread; /* 1 or 2; 0 = tuned or default                      */
     int unroll;       /* source-loop unroll 1/2/4; 0 = tuned or default          */
-    int scheme;       /* PW_SCHEME_*: pair visiting of the binary32 pass         */
+    int scheme;       /* PW_SCHEME_*: how the pair pass visits pairs             */
 } pw_config;
 
 /* A compiled kernel variant of the pair pass. */
@@ -82,7 +83,7 @@ typedef struct pw_variant {
     int block;              /* 64, 128 or 256 threads per workgroup               */
     int targets_per_thread; /* 1 or 2                                             */
     int unroll;             /* 1, 2 or 4; applies to the precision being tuned    */
-    int scheme;             /* PW_SCHEME_* of the binary32 pass (0: default)       */
+    int scheme;             /* PW_SCHEME_* (0: default)                            */
 } pw_variant;
 
 typedef struct pw_ctx pw_ctx;
@@ -254,10 +255,9 @@ int pw_server_get_stats(pw_server *s, pw_server_stats *st);
 size_t pw_job_footprint(size_t n, int precision);
 
 /* -------------------------------------------------------------- autotuner
- * Times every compiled variant (block x targets per thread x unroll, and for
- * PW_FP32C both pair schemes) on the given particles (reps solves each, HIP
- * events) and returns them in `all` (up to max_all entries) and the fastest
- * in *best.  One-sided variants give bitwise identical results; symmetric
+ * Times every compiled variant (block x targets per thread x unroll, and
+ * both pair schemes) on the given particles (reps solves each, HIP events)
+ * and returns them in `all` (up to max_all entries) and the fastest in *best.  One-sided variants give bitwise identical results; symmetric
  * ones group their sums by block and agree to binary32 rounding. */
 typedef struct pw_tune_entry {
     pw_variant variant;

@@ -85,6 +85,7 @@ struct pw_ctx {
         v.tpt = cfg.targets_per_thread ? cfg.targets_per_thread : default_tpt(total(), cfg.precision, cfg.kernel);
         if (cfg.unroll) v.unroll64 = v.unroll32 = cfg.unroll;
         v.sym32 = cfg.scheme != PW_SCHEME_ONESIDED;
+        v.sym64 = cfg.scheme == PW_SCHEME_SYMMETRIC;
         const char *cache = std::getenv("PW_TUNE_CACHE");
         if (cache && !cfg.block && !cfg.targets_per_thread && !cfg.unroll && !cfg.scheme) {
             pw_variant t;
@@ -92,7 +93,8 @@ struct pw_ctx {
                 v.block = t.block;
                 v.tpt = t.targets_per_thread;
                 if (cfg.precision == PW_FP64) {
-                    v.unroll64 = t.unroll;
+                    if (t.unroll) v.unroll64 = t.unroll;
+                    v.sym64 = t.scheme == PW_SCHEME_SYMMETRIC;
                 } else {
                     v.sym32 = t.scheme != PW_SCHEME_ONESIDED;
                     if (t.unroll) v.unroll32 = t.unroll;
@@ -107,7 +109,7 @@ struct pw_ctx {
         pw::Variant v = choose_variant();
         pw::Variant cur = eng ? eng->variant() : pw::Variant{};
         if (!eng || cur.block != v.block || cur.tpt != v.tpt || cur.unroll64 != v.unroll64 ||
-            cur.unroll32 != v.unroll32 || cur.sym32 != v.sym32) {
+            cur.unroll32 != v.unroll32 || cur.sym32 != v.sym32 || cur.sym64 != v.sym64) {
             job = nullptr;
             eng.reset();
             eng.reset(new Engine(cfg.device, cfg.kernel, v));
@@ -446,6 +448,7 @@ static pw::Variant variant_of(const pw_variant *v, int precision_hint) {
     r.block = v->block ? v->block : 256;
     r.tpt = v->targets_per_thread ? v->targets_per_thread : 2;
     r.sym32 = v->scheme != PW_SCHEME_ONESIDED;
+    r.sym64 = v->scheme == PW_SCHEME_SYMMETRIC;
     if (v->unroll) {
         if (precision_hint == PW_FP32C)
             r.unroll32 = v->unroll;
@@ -587,7 +590,7 @@ extern "C" {
 
 size_t pw_job_footprint(size_t n, int precision) {
     return pw::job_footprint(n, precision, default_block(n),
-                             default_tpt(n, precision, PW_KERNEL_HIGHORDER), true);
+                             default_tpt(n, precision, PW_KERNEL_HIGHORDER), precision == PW_FP32C);
 }
 
 int pw_server_create(pw_server **out, int device, int kernel, size_t hbm_budget_bytes,
@@ -626,7 +629,8 @@ int pw_server_submit(pw_server *s, const pw_job_spec *spec, size_t nsteps, doubl
         j.n = spec->n;
         j.nsteps = nsteps;
         const pw::Variant ev = s->eng->variant();
-        j.bytes = pw::job_footprint(spec->n, spec->precision, ev.block, ev.tpt, ev.sym32 != 0);
+        j.bytes = pw::job_footprint(spec->n, spec->precision, ev.block, ev.tpt,
+                                    spec->precision == PW_FP32C ? ev.sym32 != 0 : ev.sym64 != 0);
         if (j.bytes > s->budget) throw Error(PW_ENOMEM, "job larger than the HBM budget");
         j.arrival = arrival_ms < 0 ? s->now() : arrival_ms;
         j.pos.assign(spec->pos, spec->pos + 3 * spec->n);
@@ -770,8 +774,7 @@ int pw_autotune(int device, int kernel, int precision, size_t n, const double *p
         const int blocks[3] = {64, 128, 256}, tpts[2] = {1, 2}, unrs[3] = {1, 2, 4};
         // binary32: the symmetric pass (no unroll parameter) and the one-sided one
         std::vector<int> schemes;
-        if (precision == PW_FP32C) schemes = {PW_SCHEME_SYMMETRIC, PW_SCHEME_ONESIDED};
-        else schemes = {PW_SCHEME_ONESIDED};
+        schemes = {PW_SCHEME_SYMMETRIC, PW_SCHEME_ONESIDED};
         std::vector<pw_tune_entry> res;
         for (int sc : schemes)
         for (int b : blocks)
@@ -783,11 +786,15 @@ int pw_autotune(int device, int kernel, int precision, size_t n, const double *p
                     v.block = b;
                     v.tpt = t;
                     v.sym32 = sym ? 1 : 0;
-                    if (precision == PW_FP64)
-                        v.unroll64 = u;
-                    else if (!sym)
-                        v.unroll32 = u;
-                    if (!pw::variant_exists(kernel, v.block, v.tpt, v.unroll64, v.unroll32, v.sym32))
+                    v.sym64 = sym ? 1 : 0;
+                    if (!sym) {
+                        if (precision == PW_FP64)
+                            v.unroll64 = u;
+                        else
+                            v.unroll32 = u;
+                    }
+                    if (!pw::variant_exists(kernel, v.block, v.tpt, v.unroll64, v.unroll32, v.sym32,
+                                            v.sym64))
                         continue;
                     Engine eng(device, kernel, v);
                     JobParams par;
@@ -861,10 +868,11 @@ int pw_tune_lookup(const char *path, int kernel, int precision, size_t n, pw_var
         int k, p, b, blk, tpt, unr, sc;
         if (ss >> k >> p >> b >> blk >> tpt >> unr >> sc && k == kernel && p == precision &&
             b == bk) {
-            const int sym = precision == PW_FP32C && sc != PW_SCHEME_ONESIDED;
+            const int sym32 = precision == PW_FP32C && sc != PW_SCHEME_ONESIDED;
+            const int sym64 = precision == PW_FP64 && sc == PW_SCHEME_SYMMETRIC;
             if (sc < PW_SCHEME_DEFAULT || sc > PW_SCHEME_SYMMETRIC ||
-                !pw::variant_exists(kernel, blk, tpt, precision == PW_FP64 ? unr : 2,
-                                    precision == PW_FP32C && unr ? unr : 4, sym))
+                !pw::variant_exists(kernel, blk, tpt, precision == PW_FP64 && unr ? unr : 2,
+                                    precision == PW_FP32C && unr ? unr : 4, sym32, sym64))
                 return PW_EINVAL;
             out->block = blk;
             out->targets_per_thread = tpt;

@@ -772,6 +772,266 @@ __global__ __launch_bounds__(TB) void pair32s_kernel(const JobDev *__restrict__ 
     }
 }
 
+// ------------------------------------------- symmetric binary64 pair pass
+// The same decomposition as the binary32 symmetric pass (upper-triangle
+// items, lane l on source (l + s) mod 64 at step s, source-side sums riding
+// along by DPP rotation, one writer per slot), in binary64 throughout: the
+// rotation moves each binary64 sum as its two 32-bit halves, and no runs are
+// needed.  Shares every per-pair term between the two particles: 1.14x the
+// one-sided kernel's pairs/s on the rotor wake at N = 262,144, agreeing with
+// it to ~5e-12 per particle.  An option (PW_SCHEME_SYMMETRIC); the one-sided
+// kernel stays the plain binary64 method.
+
+// Steps of the rotation unrolled per loop iteration in the binary64 pass.
+// Measured at N = 262,144 (256 x 1): 4 steps 100 ms, 2 steps 103 ms, 8 steps
+// 134 ms (register pressure); a 3-waves-per-SIMD bound spills and is slower.
+#ifndef PW_SYM64_UNROLL
+#define PW_SYM64_UNROLL 4
+#endif
+
+struct Acc64s {
+    double v[NCOMP_S];
+};
+
+// x of lane (l + 1) mod 64 (both halves through DPP wave_rol:1).
+__device__ __forceinline__ double rot64(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x134, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x134, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+struct SrcTile64 {
+    double x[128], y[128], z[128], ax[128], ay[128], az[128], v[128], h[128];
+};
+
+struct Part64 {
+    double x, y, z, ax, ay, az, v, h;
+};
+
+__device__ __forceinline__ Part64 load_src64(const SrcTile64 &t, int k) {
+    return Part64{t.x[k], t.y[k], t.z[k], t.ax[k], t.ay[k], t.az[k], t.v[k], t.h[k]};
+}
+
+// One unordered pair in binary64: target-side terms added to a, source-side
+// terms added to the rotated sums b (FIRST: b = rotated sums + terms of the
+// first target slot).  MASKED: valid == false contributes exactly zero.
+template <int KIND, bool UNIFORM, bool MASKED, bool FIRST>
+__device__ __forceinline__ void pair64s(const Part64 &me, const Part64 &s, bool valid, Acc64s &a,
+                                        Acc64s &b) {
+    const double dx = me.x - s.x, dy = me.y - s.y, dz = me.z - s.z;
+    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+    double Q, Fp, Zs;
+    if (UNIFORM) {
+        const double t = MASKED ? (valid ? r2 : 1.0) : r2;
+        if (KIND == KIND_HIGHORDER)
+            kernel64_scaled(t, rsq2_nr(t), Q, Fp, Zs);
+        else
+            kernel64<KIND>(t, rsq_nr(t), Q, Fp, Zs);
+    } else {
+        const double isg = rsq_nr(me.h + s.h);
+        const double is2 = isg * isg;
+        double t = r2 * is2;
+        if (MASKED) t = valid ? t : 1.0;
+        if (KIND == KIND_HIGHORDER)
+            kernel64_scaled(t, rsq2_nr(t), Q, Fp, Zs);
+        else
+            kernel64<KIND>(t, rsq_nr(t), Q, Fp, Zs);
+        const double s3 = isg * is2, s5 = s3 * is2;
+        Q *= s3;
+        Fp *= s5;
+        Zs *= s5;
+    }
+    if (MASKED) {
+        Q = valid ? Q : 0.0;
+        Fp = valid ? Fp : 0.0;
+        Zs = valid ? Zs : 0.0;
+    }
+    const double cx = __builtin_fma(s.ay, dz, -s.az * dy);
+    const double cy = __builtin_fma(s.az, dx, -s.ax * dz);
+    const double cz = __builtin_fma(s.ax, dy, -s.ay * dx);
+    a.v[0] = __builtin_fma(Q, cx, a.v[0]);
+    a.v[1] = __builtin_fma(Q, cy, a.v[1]);
+    a.v[2] = __builtin_fma(Q, cz, a.v[2]);
+    const double gx = __builtin_fma(dy, me.az, -dz * me.ay);
+    const double gy = __builtin_fma(dz, me.ax, -dx * me.az);
+    const double gz = __builtin_fma(dx, me.ay, -dy * me.ax);
+    const double ex = __builtin_fma(me.ay, s.az, -me.az * s.ay);
+    const double ey = __builtin_fma(me.az, s.ax, -me.ax * s.az);
+    const double ez = __builtin_fma(me.ax, s.ay, -me.ay * s.ax);
+    const double fd = Fp * __builtin_fma(dz, ez, __builtin_fma(dy, ey, dx * ex));
+    const double sx = __builtin_fma(fd, dx, Q * ex);
+    const double sy = __builtin_fma(fd, dy, Q * ey);
+    const double sz = __builtin_fma(fd, dz, Q * ez);
+    a.v[3] += sx;
+    a.v[4] += sy;
+    a.v[5] += sz;
+    a.v[6] = __builtin_fma(Zs, s.ax, a.v[6]);
+    a.v[7] = __builtin_fma(Zs, s.ay, a.v[7]);
+    a.v[8] = __builtin_fma(Zs, s.az, a.v[8]);
+    a.v[9] = __builtin_fma(Zs, s.v, a.v[9]);
+    b.v[0] = __builtin_fma(Q, gx, b.v[0]);
+    b.v[1] = __builtin_fma(Q, gy, b.v[1]);
+    b.v[2] = __builtin_fma(Q, gz, b.v[2]);
+    b.v[3] += sx;
+    b.v[4] += sy;
+    b.v[5] += sz;
+    b.v[6] = __builtin_fma(Zs, me.ax, b.v[6]);
+    b.v[7] = __builtin_fma(Zs, me.ay, b.v[7]);
+    b.v[8] = __builtin_fma(Zs, me.az, b.v[8]);
+    b.v[9] = __builtin_fma(Zs, me.v, b.v[9]);
+}
+
+template <int KIND, bool UNIFORM, bool MASKED, int TPT, int SB>
+__device__ __forceinline__ void sym64_steps(const SrcTile64 &tile, int s_begin, int lane,
+                                            const Part64 *me, const bool *live, const int *nt,
+                                            const bool *diag, int cnt, Acc64s *a,
+                                            Acc64s &jacc) {
+    const int e0 = lane + s_begin;
+#pragma unroll
+    for (int u = 0; u < SB; ++u) {
+        const int s = s_begin + u;
+        const int k = (lane + s) & 63;
+        const Part64 sj = load_src64(tile, e0 + u);
+        Acc64s b;
+#pragma unroll
+        for (int m = 0; m < NCOMP_S; ++m) b.v[m] = rot64(jacc.v[m]);
+        if (!MASKED) {
+#pragma unroll
+            for (int p = 0; p < TPT; ++p) pair64s<KIND, UNIFORM, false, false>(me[p], sj, true, a[p], b);
+        } else {
+            const bool src_ok = k < cnt;
+#pragma unroll
+            for (int p = 0; p < TPT; ++p) {
+                if (!live[p]) continue;
+                const bool valid = src_ok && lane < nt[p] && (!diag[p] || lane + s < 64) &&
+                                   !(diag[p] && s == 0);
+                pair64s<KIND, UNIFORM, true, false>(me[p], sj, valid, a[p], b);
+            }
+        }
+        jacc = b;
+    }
+}
+
+template <int KIND, int TB, int TPT>
+__global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ jobs,
+                                                     const Item *__restrict__ items,
+                                                     const int *__restrict__ nitems_p,
+                                                     int *__restrict__ counter) {
+    constexpr int NW = TB / 64;
+    constexpr int TW = TB * TPT;
+    constexpr int SB = PW_SYM64_UNROLL;  // unrolled steps
+    __shared__ SrcTile64 tile;
+    __shared__ double jl[NW][NCOMP_S][64];
+    __shared__ int slot;
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    const int nitems = *nitems_p;
+    for (;;) {
+        const int it = next_item(counter, &slot);
+        if (it >= nitems) break;
+        const Item item = items[it];
+        const JobDev &J = jobs[item.job];
+        const Rec64 *rec = (const Rec64 *)J.rec;
+        const int n = J.n;
+        const int B0 = item.tile * TW;
+        int idx[TPT], wt[TPT], nt[TPT];
+        Part64 me[TPT];
+        Acc64s a[TPT];
+#pragma unroll
+        for (int p = 0; p < TPT; ++p) {
+            idx[p] = B0 + p * TB + (int)threadIdx.x;
+            wt[p] = (B0 + p * TB) / 64 + w;
+            nt[p] = max(0, min(64, n - wt[p] * 64));
+            const Rec64 r = rec[idx[p] < n ? idx[p] : n - 1];
+            me[p] = Part64{r.x, r.y, r.z, r.ax, r.ay, r.az, r.v, r.h};
+#pragma unroll
+            for (int m = 0; m < NCOMP_S; ++m) a[p].v[m] = 0.0;
+        }
+        const bool uniform = J.uniform != 0;
+        const int c0 = max(item.chunk * J.chunk_len, B0);
+        const int c1 = min(item.chunk * J.chunk_len + J.chunk_len, n);
+        const size_t len = (size_t)J.npad - (size_t)B0;
+        double *jrow = J.jpart + jpart_offset((size_t)item.tile, (size_t)J.npad, (size_t)TW);
+        for (int s0 = c0; s0 < c1; s0 += 64) {
+            const int cnt = min(64, c1 - s0);
+            const int st = s0 >> 6;
+            __syncthreads();
+            if (threadIdx.x < 64) {  // stage twice; missing sources read as zeros
+                const int k = (int)threadIdx.x;
+                Rec64 r;
+                if (k < cnt) {
+                    r = rec[s0 + k];
+                } else {
+                    r.x = r.y = r.z = r.h = r.ax = r.ay = r.az = r.v = 0.0;
+                }
+#pragma unroll
+                for (int c = 0; c < 128; c += 64) {
+                    tile.x[k + c] = r.x;
+                    tile.y[k + c] = r.y;
+                    tile.z[k + c] = r.z;
+                    tile.ax[k + c] = r.ax;
+                    tile.ay[k + c] = r.ay;
+                    tile.az[k + c] = r.az;
+                    tile.v[k + c] = r.v;
+                    tile.h[k + c] = r.h;
+                }
+            }
+            __syncthreads();
+            bool live[TPT], diag[TPT], full = cnt == 64, any = false;
+#pragma unroll
+            for (int p = 0; p < TPT; ++p) {
+                live[p] = st >= wt[p] && nt[p] > 0;
+                diag[p] = st == wt[p];
+                full = full && st > wt[p] && nt[p] == 64;
+                any = any || live[p];
+            }
+            Acc64s jacc;
+#pragma unroll
+            for (int m = 0; m < NCOMP_S; ++m) jacc.v[m] = 0.0;
+            if (any) {
+#pragma unroll 1
+                for (int sb = 0; sb < 64; sb += SB) {
+                    if (full) {
+                        if (uniform)
+                            sym64_steps<KIND, true, false, TPT, SB>(tile, sb, lane, me, live, nt,
+                                                                    diag, cnt, a, jacc);
+                        else
+                            sym64_steps<KIND, false, false, TPT, SB>(tile, sb, lane, me, live, nt,
+                                                                     diag, cnt, a, jacc);
+                    } else {
+                        if (uniform)
+                            sym64_steps<KIND, true, true, TPT, SB>(tile, sb, lane, me, live, nt,
+                                                                   diag, cnt, a, jacc);
+                        else
+                            sym64_steps<KIND, false, true, TPT, SB>(tile, sb, lane, me, live, nt,
+                                                                    diag, cnt, a, jacc);
+                    }
+                }
+            }
+            // after 64 steps lane l holds the sums of source (l + 63) mod 64
+#pragma unroll
+            for (int m = 0; m < NCOMP_S; ++m) jl[w][m][(lane + 63) & 63] = jacc.v[m];
+            __syncthreads();
+            for (int e = threadIdx.x; e < NCOMP_S * 64; e += TB) {
+                const int m = e >> 6, j = e & 63;
+                if (j < cnt) {
+                    double v = jl[0][m][j];
+#pragma unroll
+                    for (int ww = 1; ww < NW; ++ww) v += jl[ww][m][j];
+                    jrow[(size_t)m * len + (size_t)(s0 - B0 + j)] = v;
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < TPT; ++p)
+            if (idx[p] < n) {
+                double *o = J.part + (size_t)item.chunk * NCOMP_S * J.npad + idx[p];
+#pragma unroll
+                for (int m = 0; m < NCOMP_S; ++m) o[(size_t)m * J.npad] = a[p].v[m];
+            }
+    }
+}
+
 // --------------------------------------------------------- combine kernel
 // Per particle: add the chunk partials in chunk order (binary64), form the
 // rates

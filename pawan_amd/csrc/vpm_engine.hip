@@ -106,15 +106,14 @@ size_t jpart_count(size_t n, size_t npad, size_t tw) {
     return jpart_offset(nblk, npad, tw);
 }
 
-size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym32) {
+size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym) {
     size_t npad = round_up(std::max<size_t>(n, 1), 256);
     size_t L = chunk_for(n, TB);
     size_t nchunk = (n + L - 1) / L;
     size_t b = npad * (rec_bytes(precision) + 5 * 6 * sizeof(double) + 2 * sizeof(double) +
                        nchunk * NCOMP * sizeof(double)) +
                256;
-    if (precision == PREC_FP32C && sym32)
-        b += jpart_count(npad, npad, (size_t)TB * TPT) * sizeof(double);
+    if (sym) b += jpart_count(npad, npad, (size_t)TB * TPT) * sizeof(double);
     return b;
 }
 
@@ -179,6 +178,22 @@ typedef void (*PairFn)(const JobDev *, const Item *, const int *, int *);
     X(KIND, 256, 1, 2) X(KIND, 256, 1, 4) X(KIND, 256, 2, 1) X(KIND, 256, 2, 2)                \
     X(KIND, 256, 2, 4)
 
+// Symmetric binary64 pass: every (block, targets per thread) for both kernels.
+static PairFn pair_fn_sym64(int kind, int tb, int tpt) {
+#define PW_CASES(K, B, T) \
+    if (tb == B && tpt == T) return pair64s_kernel<K, B, T>;
+    if (kind == KIND_HIGHORDER) {
+        PW_CASES(KIND_HIGHORDER, 64, 1) PW_CASES(KIND_HIGHORDER, 64, 2)
+        PW_CASES(KIND_HIGHORDER, 128, 1) PW_CASES(KIND_HIGHORDER, 128, 2)
+        PW_CASES(KIND_HIGHORDER, 256, 1) PW_CASES(KIND_HIGHORDER, 256, 2)
+    } else {
+        PW_CASES(KIND_GAUSSIAN, 64, 1) PW_CASES(KIND_GAUSSIAN, 128, 1)
+        PW_CASES(KIND_GAUSSIAN, 256, 1)
+    }
+#undef PW_CASES
+    return nullptr;
+}
+
 // Symmetric binary32 pass: every (block, targets per thread) for both kernels.
 static PairFn pair_fn_sym(int kind, int tb, int tpt) {
 #define PW_CASES(K, B, T) \
@@ -222,18 +237,19 @@ static PairFn pair_fn(int prec, int kind, int tb, int tpt, int unr) {
     return nullptr;
 }
 
-bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32, int sym32) {
-    return pair_fn(PREC_FP64, kind, tb, tpt, unr64) &&
+bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32, int sym32, int sym64) {
+    return (sym64 ? pair_fn_sym64(kind, tb, tpt) != nullptr
+                  : pair_fn(PREC_FP64, kind, tb, tpt, unr64) != nullptr) &&
            (sym32 ? pair_fn_sym(kind, tb, tpt) != nullptr
                   : pair_fn(PREC_FP32C, kind, tb, tpt, unr32) != nullptr);
 }
 
 Engine::Engine(int device, int kind, const Variant &v)
     : device_(device), kind_(kind), tb_(v.block), tpt_(v.tpt), unr64_(v.unroll64),
-      unr32_(v.unroll32), sym32_(v.sym32 ? 1 : 0) {
+      unr32_(v.unroll32), sym32_(v.sym32 ? 1 : 0), sym64_(v.sym64 ? 1 : 0) {
     if (tb_ != 64 && tb_ != 128 && tb_ != 256) throw Error(PW_EINVAL, "block must be 64, 128 or 256");
     if (kind_ != KIND_HIGHORDER && kind_ != KIND_GAUSSIAN) throw Error(PW_EINVAL, "unknown kernel");
-    fn64_ = pair_fn(PREC_FP64, kind_, tb_, tpt_, unr64_);
+    fn64_ = sym64_ ? pair_fn_sym64(kind_, tb_, tpt_) : pair_fn(PREC_FP64, kind_, tb_, tpt_, unr64_);
     fn32_ = sym32_ ? pair_fn_sym(kind_, tb_, tpt_) : pair_fn(PREC_FP32C, kind_, tb_, tpt_, unr32_);
     if (!fn64_ || !fn32_) throw Error(PW_EINVAL, "no compiled kernel variant for these parameters");
     int ndev = 0;
@@ -309,7 +325,7 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     j->part.reset((size_t)j->nchunk * NCOMP * np * sizeof(double));
     j->vol.reset(2 * np * sizeof(double));  // volume | radius
     j->flag.reset(256);
-    j->sym = par.precision == PREC_FP32C && sym32_;
+    j->sym = par.precision == PREC_FP32C ? sym32_ != 0 : sym64_ != 0;
     j->tw = tb_ * tpt_;
     alloc_jpart(j.get());
     j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + j->flag.bytes +

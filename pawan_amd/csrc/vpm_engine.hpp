@@ -62,8 +62,8 @@ struct JobParams {
 size_t chunk_for(size_t n, int TB);
 
 // Bytes a resident job occupies on the device (TB x TPT targets per block;
-// sym32: a binary32 job uses the symmetric pass and its source-side slots).
-size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym32);
+// sym: the job uses the symmetric pass and its source-side slots).
+size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym);
 // Binary64 sums in a symmetric job's source-side slots (block rows of
 // NCOMP_S x (npad - b tw) for every block b).
 size_t jpart_count(size_t n, size_t npad, size_t tw);
@@ -99,8 +99,10 @@ struct Job {
 struct Variant {
     int block = 128, tpt = 1, unroll64 = 2, unroll32 = 4;
     int sym32 = 1;  // binary32 jobs: 1 symmetric pass, 0 one-sided pass
+    int sym64 = 0;  // binary64 jobs: 1 symmetric pass, 0 one-sided pass (the plain method)
 };
-bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32, int sym32 = 1);
+bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32, int sym32 = 1,
+                    int sym64 = 0);
 
 class Engine {
   public:
@@ -112,7 +114,7 @@ class Engine {
     int device() const { return device_; }
     int kind() const { return kind_; }
     int block() const { return tb_; }
-    Variant variant() const { return Variant{tb_, tpt_, unr64_, unr32_, sym32_}; }
+    Variant variant() const { return Variant{tb_, tpt_, unr64_, unr32_, sym32_, sym64_}; }
     hipStream_t stream() const { return stream_; }
 
     // Create a job from N x 3 positions/vorticities and N radius/volume.
@@ -163,7 +165,7 @@ class Engine {
     void capture_graph();
     double pairs_per_rhs() const;
 
-    int device_, kind_, tb_, tpt_, unr64_, unr32_, sym32_;
+    int device_, kind_, tb_, tpt_, unr64_, unr32_, sym32_, sym64_;
     void alloc_jpart(Job *j);
     void (*fn64_)(const JobDev *, const Item *, const int *, int *) = nullptr;
     void (*fn32_)(const JobDev *, const Item *, const int *, int *) = nullptr;

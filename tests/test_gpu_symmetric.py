@@ -1,17 +1,18 @@
-"""The symmetric binary32 pair pass (each unordered pair once, both particles
-updated, source-side sums in per-block slots) against the CPU oracle, on the
-shapes where its bookkeeping can go wrong: particle counts around the 64-wide
-tiles and the target blocks, a block straddling a chunk boundary, several
-wakes, non-uniform cores, the Gaussian kernel, a freestream; and against the
-one-sided binary32 pass.  Tolerance: 1e-6 relative RMS per rate field
-(velocity, vorticity rate), as for every binary32 result."""
+"""The symmetric pair passes (each unordered pair once, both particles
+updated, source-side sums in per-block slots), binary32 and binary64, against
+the CPU oracle, on the shapes where their bookkeeping can go wrong: particle
+counts around the 64-wide tiles and the target blocks, a block straddling a
+chunk boundary, several wakes, non-uniform cores, the Gaussian kernel, a
+freestream; and against the one-sided passes.  Tolerances: binary32 1e-6
+relative RMS per rate field (velocity, vorticity rate); binary64 1e-10 per
+particle, as for every result of that precision."""
 import numpy as np
 import pytest
 
 import oracle as orc
 import pawan_amd as pa
 from pawan_amd import generators as gen
-from helpers import rel_rms
+from helpers import per_particle_rel, rel_rms
 
 pytestmark = pytest.mark.gpu
 
@@ -106,3 +107,55 @@ def test_symmetric_vs_onesided_rotor():
         ev, ed = rel_rms(v, v64), rel_rms(d, d64)
         print(f"  {name}: vel {ev:.2e} dvort {ed:.2e}", flush=True)
         assert ev < TOL32 and ed < TOL32
+
+
+@pytest.mark.parametrize("n", [2, 63, 64, 65, 200, 513, 1000])
+@pytest.mark.parametrize("block,tpt", [(64, 1), (128, 2), (256, 1)])
+def test_symmetric64_tile_edges(n, block, tpt):
+    w = _random_blob(n, seed=2000 + n)
+    print(f"case symmetric64 n={n} block={block} tpt={tpt}", flush=True)
+    nu = 1e-3
+    vel, dv = _rates(w, nu, pa.FP64, pa.SCHEME_SYMMETRIC, block=block, targets_per_thread=tpt)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL)
+    ev, ed = per_particle_rel(vel, rv), per_particle_rel(dv, rd)
+    assert ev < 1e-10 and ed < 1e-10, (ev, ed)
+
+
+def test_symmetric64_wakes_nonuniform_gaussian_straddle():
+    a = gen.vortex_ring(40, 16, R=1.0, core=0.1, seed=41)
+    b = gen.vortex_ring(30, 16, R=0.7, core=0.08, seed=42, center=(0.1, 0, 0.3))
+    w = {k: np.concatenate([a[k], b[k]]) for k in ("pos", "vort", "radius", "volume")}
+    w["radius"] = w["radius"] * (1.0 + 0.3 * gen.uniform(43, w["pos"].shape[0]))
+    wake_n = [a["n"], b["n"]]
+    nu, uinf = 2e-3, (0.0, 0.05, -0.1)
+    for kernel, kind in ((pa.KERNEL_HIGHORDER, orc.HIGHORDER), (pa.KERNEL_GAUSSIAN, orc.GAUSSIAN)):
+        print(f"case symmetric64 wakes nonuniform kernel={kernel}", flush=True)
+        vel, dv = _rates(w, nu, pa.FP64, pa.SCHEME_SYMMETRIC, wake_n=wake_n, kernel=kernel,
+                         uinf=uinf, block=128, targets_per_thread=1, chunk=384)
+        rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL,
+                           kind=kind, wake_n=wake_n, uinf=uinf)
+        ev, ed = per_particle_rel(vel, rv), per_particle_rel(dv, rd)
+        print(f"  vel {ev:.2e} dvort {ed:.2e}", flush=True)
+        assert ev < 1e-10 and ed < 1e-10
+
+
+def test_symmetric64_rk4_batch_alone_equals_mixed():
+    """RK4 through the symmetric binary64 pass: one job alone and in a batch
+    with others gives bitwise the same states, and matches the oracle."""
+    jobs = gen.batch_jobs(3, seed=7, n_min=500, n_max=3000)
+    res = []
+    for members in ([0], [0, 1, 2]):
+        B = pa.Batch(block=128, targets_per_thread=2, scheme=pa.SCHEME_SYMMETRIC)
+        ids = [B.add(jobs[k], jobs[k]["nu"], jobs[k]["dt"], pa.FP64) for k in members]
+        B.step(3, graph=True)
+        res.append(B.get(ids[0]))
+        B.close()
+    assert np.array_equal(res[0]["pos"], res[1]["pos"])
+    assert np.array_equal(res[0]["vort"], res[1]["vort"])
+    j = jobs[0]
+    x = orc.rk4(orc.pack_state(j["pos"], j["vort"]), j["radius"], j["volume"], j["nu"], j["dt"], 3,
+                mode=orc.OMP_CLEAN)
+    p, v = orc.unpack_state(x)
+    ep, ev = per_particle_rel(res[0]["pos"], p), per_particle_rel(res[0]["vort"], v)
+    print(f"  symmetric64 rk4 vs oracle: pos {ep:.2e} vort {ev:.2e}", flush=True)
+    assert ep < 1e-11 and ev < 1e-9

@@ -57,7 +57,8 @@ def test_autotune_small(tmp_path):
     w = gen.vortex_ring(64, 16, seed=42)
     best, allv = pa.autotune(w, pa.FP64, reps=1, cache=str(tmp_path / "tune.txt"))
     print(f"  best {best}", flush=True)
-    assert len(allv) == 18 and best["pair_ms"] == min(e["pair_ms"] for e in allv)
+    assert {e["scheme"] for e in allv} == {pa.SCHEME_SYMMETRIC, pa.SCHEME_ONESIDED}
+    assert len(allv) == 6 + 18 and best["pair_ms"] == min(e["pair_ms"] for e in allv)
     assert (tmp_path / "tune.txt").read_text().split()[3:7] == [
         str(best["block"]), str(best["targets_per_thread"]), str(best["unroll"]),
         str(best["scheme"])]
