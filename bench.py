@@ -289,14 +289,24 @@ def bench_leapfrog(pa, gen, args):
         rates[name] = S.getRates()
         d0 = S.diagnose()
         t0 = time.perf_counter()
-        S.rk4(w["dt"], steps)
+        nonfinite = False
+        try:
+            S.rk4(w["dt"], steps)  # every step runs; a non-finite state is reported at the end
+        except pa.PawanError as e:
+            if e.code != 4:
+                raise
+            nonfinite = True
         sec = time.perf_counter() - t0
         log(f"  leapfrog {name}: {steps} steps in {sec:.1f} s")
-        d1 = S.diagnose()
         final[name] = S.getStates()
+        res = {"value": 4.0 * steps * n * (n - 1) / sec, "ms_per_step": sec * 1e3 / steps,
+               "non_finite": nonfinite}
+        fin = np.isfinite(final[name])
+        res["max_abs_finite_state_end"] = float(np.abs(final[name][fin]).max()) if fin.any() else None
+        if not nonfinite:
+            res["drift"] = _drift(d0, S.diagnose())
         S.close()
-        out[name] = {"value": 4.0 * steps * n * (n - 1) / sec, "ms_per_step": sec * 1e3 / steps,
-                     "drift": _drift(d0, d1)}
+        out[name] = res
     out["value"] = out["fp64"]["value"]
     out["dtype"] = "f64"
     r64, r32 = rates["fp64"], rates["fp32c"]
@@ -304,12 +314,16 @@ def bench_leapfrog(pa, gen, args):
     v = np.r_[r64[:3 * n1], r64[6 * n1:6 * n1 + 3 * (n - n1)]]
     v32 = np.r_[r32[:3 * n1], r32[6 * n1:6 * n1 + 3 * (n - n1)]]
     out["fp32c"]["rel_rms_rate_err_vs_fp64_t0"] = {"all": rr(r32, r64), "velocity": rr(v32, v)}
+    out["note"] = ("the reference HIGHORDER kernel has q ~ 1/rho^3 at small separation (the phi term "
+                   "keeps t^-1/2 while Z decays as (t+1)^-7.5); over long runs the two rings' "
+                   "particles meet and the state diverges in binary64 as in binary32")
     # state after the run: binary32 pairs vs binary64, positions and strengths
     f64, f32 = final["fp64"], final["fp32c"]
     pos = np.r_[np.arange(3 * n1), 6 * n1 + np.arange(3 * (n - n1))]
     vor = np.r_[3 * n1 + np.arange(3 * n1), 6 * n1 + 3 * (n - n1) + np.arange(3 * (n - n1))]
-    out["fp32c"]["rel_rms_state_diff_vs_fp64_end"] = {"position": rr(f32[pos], f64[pos]),
-                                                     "vorticity": rr(f32[vor], f64[vor])}
+    if np.isfinite(f64).all() and np.isfinite(f32).all():
+        out["fp32c"]["rel_rms_state_diff_vs_fp64_end"] = {"position": rr(f32[pos], f64[pos]),
+                                                         "vorticity": rr(f32[vor], f64[vor])}
     return out
 
 

@@ -814,9 +814,9 @@ __device__ __forceinline__ Part64 load_src64(const SrcTile64 &t, int k) {
 }
 
 // One unordered pair in binary64: target-side terms added to a, source-side
-// terms added to the rotated sums b (FIRST: b = rotated sums + terms of the
-// first target slot).  MASKED: valid == false contributes exactly zero.
-template <int KIND, bool UNIFORM, bool MASKED, bool FIRST>
+// terms added to b (the source's running sums, just rotated in).  MASKED:
+// valid == false contributes exactly zero.
+template <int KIND, bool UNIFORM, bool MASKED>
 __device__ __forceinline__ void pair64s(const Part64 &me, const Part64 &s, bool valid, Acc64s &a,
                                         Acc64s &b) {
     const double dx = me.x - s.x, dy = me.y - s.y, dz = me.z - s.z;
@@ -898,7 +898,7 @@ __device__ __forceinline__ void sym64_steps(const SrcTile64 &tile, int s_begin, 
         for (int m = 0; m < NCOMP_S; ++m) b.v[m] = rot64(jacc.v[m]);
         if (!MASKED) {
 #pragma unroll
-            for (int p = 0; p < TPT; ++p) pair64s<KIND, UNIFORM, false, false>(me[p], sj, true, a[p], b);
+            for (int p = 0; p < TPT; ++p) pair64s<KIND, UNIFORM, false>(me[p], sj, true, a[p], b);
         } else {
             const bool src_ok = k < cnt;
 #pragma unroll
@@ -906,7 +906,7 @@ __device__ __forceinline__ void sym64_steps(const SrcTile64 &tile, int s_begin, 
                 if (!live[p]) continue;
                 const bool valid = src_ok && lane < nt[p] && (!diag[p] || lane + s < 64) &&
                                    !(diag[p] && s == 0);
-                pair64s<KIND, UNIFORM, true, false>(me[p], sj, valid, a[p], b);
+                pair64s<KIND, UNIFORM, true>(me[p], sj, valid, a[p], b);
             }
         }
         jacc = b;
