@@ -267,6 +267,65 @@ def bench_ring1(pa, gen, args):
                              "kind": "port", "sample": "the full 100-step run"}}
 
 
+def bench_frames(pa, gen, args):
+    """A .wake frame after every RK4 step, as integrate() writes them
+    (integration.cpp:16-35): per-step time with no frames, with synchronous
+    frames (copy to the host, then fwrite, before the next step) and with
+    asynchronous frames (snapshot on the device; copy and file write overlap
+    the next step).  Files go to a temporary directory and are checked equal."""
+    import tempfile
+    cases = []
+    ring = gen.config_single_ring()
+    cases.append(("single_ring_2048", [ring], pa.FP64, ring["dt"], 100))
+    lf = gen.config_leapfrog()
+    n1 = lf["wakes"][0]
+    halves = [{k: lf[k][:n1] for k in ("pos", "vort", "radius", "volume")},
+              {k: lf[k][n1:] for k in ("pos", "vort", "radius", "volume")}]
+    cases.append(("leapfrog_65536", halves, pa.FP64, lf["dt"], 20))
+    rot = gen.config_rotor_wake(args.n_shed)
+    cases.append((f"rotor_wake_{rot['n']}_fp32c", [rot], pa.FP32C, STEP_DT, 6))
+    out = {"metric": "RK4 step time with a .wake frame per step: none / synchronous / asynchronous",
+           "unit": "ms/step", "n_gpus": 1, "cases": {}}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, wakes, prec, dt, steps in cases:
+            res = {}
+            files = {}
+            for mode in ("none", "sync", "async"):
+                S = pa.Interaction(*[pa.Wake.from_dict(w) for w in wakes], nu=1e-4, precision=prec)
+                S.rk4(dt, 1)  # warm: plan, graph, first launches
+                path = os.path.join(tmp, f"{name}_{mode}.wake")
+                f = pa.WakeFile(path)
+                t0 = time.perf_counter()
+                if mode == "sync":
+                    S.write(f, 0.0)
+                elif mode == "async":
+                    S.write_async(f, 0.0)
+                for i in range(1, steps + 1):
+                    S.rk4(dt, 1)
+                    if mode == "sync":
+                        S.write(f, i * dt)
+                    elif mode == "async":
+                        S.write_async(f, i * dt)
+                f.close()  # waits for queued frames
+                sec = time.perf_counter() - t0
+                S.close()
+                res[mode] = sec * 1e3 / steps
+                if mode != "none":
+                    with open(path, "rb") as fh:
+                        files[mode] = fh.read()
+                os.remove(path)
+            n = sum(len(w["pos"]) for w in wakes)
+            res["frame_bytes"] = len(files["sync"]) // (steps + 1)
+            res["files_identical"] = files["sync"] == files["async"]
+            res["sync_overhead"] = res["sync"] / res["none"] - 1.0
+            res["async_overhead"] = res["async"] / res["none"] - 1.0
+            res["particles"] = n
+            log(f"  frames {name}: none {res['none']:.3f} sync {res['sync']:.3f} "
+                f"async {res['async']:.3f} ms/step, identical {res['files_identical']}")
+            out["cases"][name] = res
+    return out
+
+
 def bench_leapfrog(pa, gen, args):
     """Two coaxial rings, 2 x 32,768 particles, as two wakes; binary64 and
     compensated binary32, RK4 dt 0.005 x steps (400 for the full config)."""
@@ -452,7 +511,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner; run the variants it picks on MI355X at N=262,144")
     ap.add_argument("--workload", default="rotor262k",
-                    choices=["rotor262k", "ring1", "leapfrog", "rotor_growth", "batch", "serve"])
+                    choices=["rotor262k", "ring1", "leapfrog", "rotor_growth", "batch", "serve",
+                             "frames"])
     ap.add_argument("--shed-steps", type=int, default=1024)
     ap.add_argument("--jobs", type=int, default=256)
     ap.add_argument("--rate", type=float, default=2.0, help="serving arrivals per second")
@@ -470,7 +530,7 @@ def main():
     from pawan_amd import generators as gen
 
     other = {"ring1": bench_ring1, "leapfrog": bench_leapfrog, "rotor_growth": bench_rotor_growth,
-             "batch": bench_batch, "serve": bench_serve}
+             "batch": bench_batch, "serve": bench_serve, "frames": bench_frames}
     if args.workload in other:
         print(json.dumps(other[args.workload](pa, gen, args)), flush=True)
         return

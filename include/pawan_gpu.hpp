@@ -162,11 +162,38 @@ class interaction : public Interaction, public device_system {
     std::vector<Wake *> wakes_;
 };
 
+// __integration::integrate (integration.cpp:16-35) with its frames written
+// asynchronously (pw_write_async): f receives the same bytes -- the time, then
+// S->write(f)'s frame, at t = 0 and after every step -- but a step does not
+// wait for the previous frame's copy or file write.  f stays open.
+inline void integrate_frames(device_system *d, FILE *f, double dt, size_t n, bool rk) {
+    pw_ctx *c = d->context();
+    check(pw_write_async(c, f, 0.0, 1), c);
+    for (size_t i = 1; i <= n; ++i) {
+        check(rk ? pw_rk4_steps(c, dt, 1) : pw_euler_steps(c, dt, 1), c);
+        check(pw_write_async(c, f, i * dt, 1), c);
+    }
+    check(pw_write_wait(c), c);
+    d->pull();
+}
+
+template <class System>
+device_system *device_of(System *S) {
+    device_system *d = dynamic_cast<device_system *>(S);
+    if (!d) throw std::invalid_argument("pawan_gpu integrators step a pawan_gpu::interaction");
+    return d;
+}
+
 // __rk4 whose step runs on the device (rk4.cpp:12-77 fused into pw_rk4_steps)
 template <class RK4, class System, class Vector>
 class rk4 : public RK4 {
   public:
     rk4(const double &t, const size_t &n) : RK4(t, n) {}
+    // integrate(S, IO) with f = IO->create_binary_file(".wake") and the
+    // frames written asynchronously; the caller closes f
+    void integrate_async(System *S, FILE *f) {
+        integrate_frames(device_of(S), f, this->_dt, this->_n, true);
+    }
 
   protected:
     void step(const double &dt, System *S, Vector *states) override {
@@ -182,6 +209,10 @@ template <class Integration, class System, class Vector>
 class euler : public Integration {
   public:
     euler(const double &t, const size_t &n) : Integration(t, n) {}
+    // integrate(S, IO) with asynchronous frames, as rk4::integrate_async
+    void integrate_async(System *S, FILE *f) {
+        integrate_frames(device_of(S), f, this->_dt, this->_n, false);
+    }
 
   protected:
     void step(const double &dt, System *S, Vector *states) override {

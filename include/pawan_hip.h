@@ -160,6 +160,23 @@ int pw_resolve(pw_ctx *ctx, double *field);
  * (wake.cpp:68-73); integrate() writes the time before each frame */
 int pw_write(pw_ctx *ctx, FILE *f);
 
+/* Asynchronous frames.  No reference function of its own: it replaces the
+ * per-step "fwrite(&t); S->write(f)" of __integration::integrate
+ * (integration.cpp:18-29), which stalls the stepping on the copy and the file
+ * write.  pw_write_async snapshots the current state on the device, after
+ * every step already queued, and returns; a copy stream moves the snapshot
+ * into one of two pinned host buffers and a writer thread appends to f the
+ * time t (when with_time != 0, as integrate writes it) followed by exactly
+ * the bytes pw_write would write for that state.  Frames reach f in call
+ * order; with two frames in flight a third call waits for the older one.
+ * The caller must not touch f between pw_write_async and the pw_write_wait
+ * that follows it.  A failed frame (fwrite short) is reported by the next
+ * pw_write_async (which then queues nothing) or pw_write_wait. */
+int pw_write_async(pw_ctx *ctx, FILE *f, double t, int with_time);
+/* wait until every frame queued by pw_write_async is in its file (no fflush);
+ * returns the first failure since the last wait */
+int pw_write_wait(pw_ctx *ctx);
+
 /* Particle emission: append n particles to wake `wake` (its _numParticles
  * grows; the state vector grows by 6n).  Device state of the particles
  * already present is kept.  No reference counterpart (__wake has a fixed

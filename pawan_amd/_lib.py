@@ -81,6 +81,8 @@ SIGNATURES = {
     "pw_diagnose": (C.c_int, [C.c_void_p, C.POINTER(PwDiag)]),
     "pw_resolve": (C.c_int, [C.c_void_p, _dp]),
     "pw_write": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pw_write_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int]),
+    "pw_write_wait": (C.c_int, [C.c_void_p]),
     "pw_append_particles": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, _dp, _dp, _dp, _dp]),
     "pw_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(PwVariant)]),
     "pw_batch_destroy": (C.c_int, [C.c_void_p]),

@@ -14,6 +14,7 @@
 
 #include "../../include/pawan_hip.h"
 #include "vpm_engine.hpp"
+#include "vpm_writer.hpp"
 
 using pw::Engine;
 using pw::Error;
@@ -76,6 +77,9 @@ struct pw_ctx {
     std::vector<int64_t> woff{0};
     std::vector<double> pos, vort, rad, vol;
     double dt = 0.0;
+    // asynchronous frames (pw_write_async); declared after eng so that it is
+    // destroyed first, once its frames are written
+    std::unique_ptr<pw::FrameWriter> writer;
 
     size_t total() const { return (size_t)woff.back(); }
 
@@ -107,6 +111,7 @@ struct pw_ctx {
     void rebuild() {
         if (total() == 0) throw Error(PW_EINVAL, "no particles: add a wake first");
         pw::Variant v = choose_variant();
+        if (writer) writer->quiesce();  // queued frames read the current device state
         pw::Variant cur = eng ? eng->variant() : pw::Variant{};
         if (!eng || cur.block != v.block || cur.tpt != v.tpt || cur.unroll64 != v.unroll64 ||
             cur.unroll32 != v.unroll32 || cur.sym32 != v.sym32 || cur.sym64 != v.sym64) {
@@ -190,7 +195,10 @@ int pw_create(pw_ctx **out, const pw_config *cfg) {
 
 int pw_destroy(pw_ctx *ctx) {
     if (!ctx) return PW_EINVAL;
-    int rc = guarded(nullptr, [&] { ctx->eng.reset(); });
+    int rc = guarded(nullptr, [&] {
+        ctx->writer.reset();
+        ctx->eng.reset();
+    });
     delete ctx;
     return rc;
 }
@@ -369,6 +377,31 @@ int pw_write(pw_ctx *ctx, FILE *f) {
                  std::fwrite(ctx->rad.data() + o, sizeof(double), m, f) == m;
         }
         if (!ok) throw Error(PW_EINVAL, "write to the file failed");
+    });
+}
+
+int pw_write_async(pw_ctx *ctx, FILE *f, double t, int with_time) {
+    if (!ctx || !f) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        ctx->ready();
+        if (!ctx->writer) ctx->writer.reset(new pw::FrameWriter(ctx->cfg.device));
+        std::string m;
+        if (ctx->writer->status(&m) != PW_OK) {  // an earlier frame failed: report it
+            const int code = ctx->writer->drain(&m);
+            throw Error(code, m);
+        }
+        ctx->writer->submit(ctx->eng.get(), ctx->job, f, t, with_time != 0, ctx->wake_n,
+                            ctx->rad);
+    });
+}
+
+int pw_write_wait(pw_ctx *ctx) {
+    if (!ctx) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        if (!ctx->writer) return;
+        std::string m;
+        const int code = ctx->writer->drain(&m);
+        if (code != PW_OK) throw Error(code, m);
     });
 }
 

@@ -718,6 +718,16 @@ void Engine::download_aos(Job *j, int which, double *out) {
     PW_HIP(hipStreamSynchronize(stream_));
 }
 
+void Engine::snapshot_aos(Job *j, double *dst) {
+    PW_HIP(hipSetDevice(device_));
+    const size_t np = j->npad, n = j->n;
+    JobDev d = j->dev();
+    for (int h = 0; h < 2; ++h)
+        hipLaunchKernelGGL(aos_soa_kernel, dim3(grid_for(n)), dim3(256), 0, stream_,
+                           dst + (size_t)3 * h * n, d.x0 + (size_t)3 * h * np, (int)n, (int)np, 1);
+    PW_HIP(hipGetLastError());
+}
+
 void Engine::upload_state_layout(Job *j, const double *state, const std::vector<int64_t> &woff) {
     PW_HIP(hipSetDevice(device_));
     const size_t np = j->npad, n = j->n;

@@ -142,6 +142,10 @@ class Engine {
     // Host <-> device transfers (synchronous on the engine stream).
     void upload_state_soa(Job *j, const double *pos, const double *vort);  // N x 3 each
     void download_aos(Job *j, int which, double *out);  // which: 0 pos 1 vort 2 vel 3 dvort
+    // Queue a copy of the job's positions then vorticities (N x 3 each,
+    // row-major) into device memory dst (6N doubles) on the engine stream,
+    // after every step already queued; returns without waiting.
+    void snapshot_aos(Job *j, double *dst);
     void upload_state_layout(Job *j, const double *state, const std::vector<int64_t> &woff);
     void download_state_layout(Job *j, int which, double *state,
                                const std::vector<int64_t> &woff);  // which: 0 state 1 rates

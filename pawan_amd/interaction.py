@@ -181,23 +181,30 @@ class Interaction:
 
     # interaction.cpp:197-202
     def write(self, path, t=None, append=True):
-        """Append a frame to a .wake file: the time (when given, as
-        integrate() writes it) then nWake and each wake's particles."""
-        libc = C.CDLL(None)
-        libc.fopen.restype = C.c_void_p
-        libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
-        libc.fclose.argtypes = [C.c_void_p]
-        f = libc.fopen(str(path).encode(), b"ab" if append else b"wb")
-        if not f:
-            raise OSError(f"cannot open {path}")
-        try:
+        """Append a frame to a .wake file (a path, or an open WakeFile): the
+        time (when given, as integrate() writes it) then nWake and each
+        wake's particles.  Synchronous."""
+        if isinstance(path, WakeFile):
             if t is not None:
-                libc.fwrite.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p]
-                tt = C.c_double(t)
-                libc.fwrite(C.byref(tt), 8, 1, f)
-            self._ck(lib().pw_write(self._h, f))
-        finally:
-            libc.fclose(f)
+                path.write_time(t)
+            self._ck(lib().pw_write(self._h, path.handle))
+            return
+        with WakeFile(path, append=append) as f:
+            if t is not None:
+                f.write_time(t)
+            self._ck(lib().pw_write(self._h, f.handle))
+
+    def write_async(self, f: "WakeFile", t=None):
+        """Queue a frame of the current state into the open ``f`` (time
+        first when given) and return at once; ``write_wait`` or closing ``f``
+        waits for it.  The bytes are those of ``write``."""
+        f.users.add(self)
+        self._ck(lib().pw_write_async(self._h, f.handle, 0.0 if t is None else float(t),
+                                      0 if t is None else 1))
+
+    def write_wait(self):
+        """Wait until every queued frame is in its file."""
+        self._ck(lib().pw_write_wait(self._h))
 
     def appendParticles(self, wake: int, pos, vort, radius, volume):
         """Emit particles into wake ``wake`` (the state grows by 6 per particle)."""
@@ -224,6 +231,45 @@ class Interaction:
         return {"total_ms": t.value, "pair_ms": p.value, "pairs": n.value}
 
 
+class WakeFile:
+    """An open .wake file (C stdio), for ``Interaction.write_async``.
+    Closing it first waits for the frames queued into it."""
+
+    def __init__(self, path, append: bool = False):
+        libc = C.CDLL(None)
+        libc.fopen.restype = C.c_void_p
+        libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+        libc.fclose.argtypes = [C.c_void_p]
+        libc.fwrite.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p]
+        libc.fwrite.restype = C.c_size_t
+        self._libc = libc
+        self.handle = libc.fopen(str(path).encode(), b"ab" if append else b"wb")
+        if not self.handle:
+            raise OSError(f"cannot open {path}")
+        self.users = set()
+
+    def write_time(self, t: float):
+        tt = C.c_double(t)
+        if self._libc.fwrite(C.byref(tt), 8, 1, self.handle) != 1:
+            raise OSError("write to the file failed")
+
+    def close(self):
+        if self.handle:
+            try:
+                for S in self.users:
+                    if getattr(S, "_h", None):
+                        S.write_wait()
+            finally:
+                self._libc.fclose(self.handle)
+                self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 class Integration:
     """pawan::__integration (integration.h:20-58): forward Euler stepping."""
 
@@ -234,10 +280,20 @@ class Integration:
     def step(self, dt: float, S: Interaction):
         S.euler(dt, 1)
 
-    def integrate(self, S: Interaction, writer=None):
+    def integrate(self, S: Interaction, writer=None, path=None):
         """Advance S by n steps; ``writer(t, S)`` is called at t = 0 and after
         every step, as integrate() writes a frame per step
-        (integration.cpp:16-35)."""
+        (integration.cpp:16-35).  With ``path``, the frames (time, then the
+        state) go to that .wake file asynchronously: each step is queued
+        behind the previous frame's snapshot without waiting for its copy or
+        its file write."""
+        if path is not None:
+            with WakeFile(path) as f:
+                S.write_async(f, 0.0)
+                for i in range(1, self._n + 1):
+                    self.step(self._dt, S)
+                    S.write_async(f, i * self._dt)
+            return
         t = 0.0
         if writer:
             writer(t, S)
@@ -254,11 +310,11 @@ class RK4(Integration):
     def step(self, dt: float, S: Interaction):
         S.rk4(dt, 1)
 
-    def integrate(self, S: Interaction, writer=None):
-        if writer is None:
+    def integrate(self, S: Interaction, writer=None, path=None):
+        if writer is None and path is None:
             S.rk4(self._dt, self._n)
             return
-        super().integrate(S, writer)
+        super().integrate(S, writer, path)
 
 
 class Batch:

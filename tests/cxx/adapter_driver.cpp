@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
+#include <string>
 #include <vector>
 
 #include "../../include/pawan_gpu.hpp"
@@ -133,6 +135,29 @@ int main(int argc, char **argv) {
         std::printf("driver: .wake file %ld bytes = 4 frames of %ld\n", sz, frame);
         bad |= sz != 4 * frame;
         std::fclose(f);
+        {  // the same integration with asynchronous frames: the same bytes
+            mini::wake *A1 = ring(64, 16, 1.0, 0.0), *A2 = ring(48, 16, 0.8, 0.3);
+            std::string p2 = std::string(path) + ".async";
+            {
+                gpu_interaction SA(A1, A2, o);
+                gpu_rk4 INA(0.03, 3);
+                FILE *fa = std::fopen(p2.c_str(), "wb");
+                INA.integrate_async(&SA, fa);
+                std::fclose(fa);
+            }
+            FILE *f1 = std::fopen(path, "rb"), *f2 = std::fopen(p2.c_str(), "rb");
+            std::vector<char> b1(sz), b2(sz + 1);
+            size_t r1 = std::fread(b1.data(), 1, sz, f1), r2 = std::fread(b2.data(), 1, sz + 1, f2);
+            std::fclose(f1);
+            std::fclose(f2);
+            bool eq = r1 == (size_t)sz && r2 == (size_t)sz &&
+                      std::equal(b1.begin(), b1.end(), b2.begin());
+            std::printf("driver: integrate_async() frames %s\n",
+                        eq ? "byte-identical" : "DIFFER");
+            bad |= !eq;
+            delete A1;
+            delete A2;
+        }
         double om[3], im[3], ke;
         S.device_diagnostics(om, im, &ke);
         std::printf("driver: kinetic energy %.12e\n", ke);

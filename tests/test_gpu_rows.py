@@ -152,3 +152,89 @@ def test_emission_into_first_of_two_wakes():
     vel, dv = orc.rates(p, v, rad, vol, 1e-3, orc.SERIAL, wake_n=[n1, n2])
     ref = orc.pack_state(vel, dv, [n1, n2])
     assert np.max(np.abs(r - ref)) <= 1e-10 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("precision", [pa.FP64, pa.FP32C])
+def test_async_frames_equal_sync_frames(tmp_path, precision):
+    """Frames queued with write_async (snapshot on the device, copy stream,
+    writer thread) are byte-identical to synchronous write() frames of the
+    same run, including frames after the wake grows by emission."""
+    a, b, _ = _two_wakes()
+    e = gen.vortex_ring(8, 16, seed=34, center=(0.0, 0.0, -0.4))
+    print(f"case async frames precision={precision}", flush=True)
+    S1 = pa.Interaction(pa.Wake.from_dict(a), pa.Wake.from_dict(b), nu=1e-3, precision=precision)
+    S2 = pa.Interaction(pa.Wake.from_dict(a), pa.Wake.from_dict(b), nu=1e-3, precision=precision)
+    ps, pa_ = tmp_path / "sync.wake", tmp_path / "async.wake"
+    S1.write(ps, t=0.0, append=False)
+    with pa.WakeFile(pa_) as f:
+        S2.write_async(f, 0.0)
+        for k in range(1, 7):
+            if k == 4:
+                for S in (S1, S2):
+                    S.appendParticles(1, e["pos"], e["vort"], e["radius"], e["volume"])
+            S1.rk4(0.01, 1)
+            S2.rk4(0.01, 1)
+            S1.write(ps, t=0.01 * k)
+            S2.write_async(f, 0.01 * k)
+        S2.write_async(f)  # a frame without the time, as write() without t
+    S1.write(ps)
+    st1, st2 = S1.getStates(), S2.getStates()
+    S1.close()
+    S2.close()
+    assert np.array_equal(st1, st2)
+    raw1, raw2 = ps.read_bytes(), pa_.read_bytes()
+    print(f"  {len(raw1)} bytes sync, {len(raw2)} async", flush=True)
+    assert raw1 == raw2
+
+
+def test_integrate_to_file_async(tmp_path):
+    """Integration.integrate(S, path=...) (asynchronous frames) writes what
+    integrate() with a synchronous frame writer writes."""
+    w = gen.vortex_ring(64, 16, seed=35)
+    p1, p2 = tmp_path / "a.wake", tmp_path / "b.wake"
+    S1 = pa.Interaction(pa.Wake.from_dict(w), nu=1e-3)
+    S2 = pa.Interaction(pa.Wake.from_dict(w), nu=1e-3)
+    first = [True]
+
+    def sync_writer(t, S):
+        S.write(p1, t=t, append=not first[0])
+        first[0] = False
+
+    pa.RK4(0.05, 5).integrate(S1, writer=sync_writer)
+    pa.RK4(0.05, 5).integrate(S2, path=p2)
+    x1, x2 = S1.getStates(), S2.getStates()
+    S1.close()
+    S2.close()
+    assert np.array_equal(x1, x2)
+    assert p1.read_bytes() == p2.read_bytes()
+    assert len(p2.read_bytes()) == 6 * (8 + 8 + 8 + 56 * w["n"])
+
+
+def test_async_frame_write_failure_reported(tmp_path):
+    """A frame whose file write fails (a stream opened read-only) is reported
+    by the following wait, once; the context keeps working."""
+    import ctypes as C
+    from pawan_amd import _lib
+    from pawan_amd.interaction import lib
+    w = gen.vortex_ring(16, 9, seed=36)
+    S = pa.Interaction(pa.Wake.from_dict(w), nu=1e-3)
+    p = tmp_path / "ro.wake"
+    p.write_bytes(b"")
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    f = libc.fopen(str(p).encode(), b"rb")
+    assert f
+    try:
+        assert lib().pw_write_async(S._h, f, 0.0, 1) == _lib.PW_OK
+        assert lib().pw_write_wait(S._h) == _lib.PW_EINVAL
+        assert b"write" in lib().pw_last_error(S._h)
+        assert lib().pw_write_wait(S._h) == _lib.PW_OK
+    finally:
+        libc.fclose(f)
+    good = tmp_path / "ok.wake"
+    with pa.WakeFile(good) as g:
+        S.write_async(g, 0.0)
+    S.close()
+    assert len(good.read_bytes()) == 8 + 8 + 8 + 56 * w["n"]
