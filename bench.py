@@ -145,14 +145,17 @@ def cpu_baseline(w, budget_s=15.0):
 
 
 def serving_run(pa, gen, duration_s=20.0, rate_hz=4.0, seed=1, budget=200 << 30, n_lo=2048,
-                n_hi=32768, s_lo=50, s_hi=200, drain_factor=3.0):
+                n_hi=32768, s_lo=50, s_hi=200, drain_factor=3.0, check_failed=False):
     """Poisson arrivals for duration_s; ring or rotor-wake jobs with N
     log-uniform in [n_lo, n_hi], steps uniform in [s_lo, s_hi], precision
     50/50; runs until all arrived jobs finish or drain_factor x duration_s
-    passes (jobs still running then are not counted as finished)."""
+    passes (jobs still running then are not counted as finished).
+    check_failed: rerun every failed binary32 job alone in binary64 for the
+    steps it completed and report the largest |state| it reaches (a job
+    that diverges there too diverges under the method itself)."""
     u = gen.uniform(seed, 4096)
     srv = pa.Server(budget)
-    t_next, k, ids, meta = 0.0, 0, [], {}
+    t_next, k, ids, meta, specs = 0.0, 0, [], {}, {}
     t_end = duration_s * 1000.0
     t_log = 0.0
     while True:
@@ -171,10 +174,12 @@ def serving_run(pa, gen, duration_s=20.0, rate_hz=4.0, seed=1, budget=200 << 30,
                 n_cs = gen.ring_cs_for(n, 1.0, 0.1)
                 w = gen.vortex_ring(max(8, n // n_cs), n_cs, R=1.0, core=0.1, seed=seed + k)
                 ids.append(srv.submit(w, 1e-3, 0.005, steps, prec, arrival_ms=t_next))
+                specs[ids[-1]] = (w, 1e-3, 0.005, (0.0, 0.0, 0.0))
             else:
                 w = gen.rotor_wake(max(1, n // 256))
                 ids.append(srv.submit(w, 1e-4, LONG_DT, steps, prec, uinf=(0, 0, -0.05),
                                       arrival_ms=t_next))
+                specs[ids[-1]] = (w, 1e-4, LONG_DT, (0.0, 0.0, -0.05))
             meta[ids[-1]] = ("ring" if a[3] < 0.5 else "rotor", w["pos"].shape[0],
                              "fp64" if prec == pa.FP64 else "fp32c", steps)
             k += 1
@@ -193,9 +198,25 @@ def serving_run(pa, gen, duration_s=20.0, rate_hz=4.0, seed=1, budget=200 << 30,
         state, done = srv.state(i)
         if state == srv.FAILED:
             failed.append(list(meta[i]) + [done])
+            if check_failed and meta[i][2] == "fp32c":
+                w, nu, dt, uinf = specs[i]
+                S = pa.Interaction(pa.Wake.from_dict(w), nu=nu, uinf=uinf)
+                try:
+                    S.rk4(dt, done)
+                    fin = True
+                except pa.PawanError as e:
+                    if e.code != 4:
+                        raise
+                    fin = False
+                x = S.getStates()
+                S.close()
+                ok = np.isfinite(x)
+                failed[-1].append({"fp64_finite": fin, "fp64_max_abs_state": float(np.abs(x[ok]).max()),
+                                   "max_abs_state_t0": float(np.abs(w["pos"]).max())})
+                log(f"  failed {meta[i]} at step {done}: alone in binary64 {failed[-1][-1]}")
     srv.close()
     return {
-        "failed_jobs": failed,  # [kind, n, precision, steps asked, steps done]
+        "failed_jobs": failed,  # [kind, n, precision, steps asked, steps done(, binary64 check)]
         "arrival_rate_hz": rate_hz, "duration_s": duration_s, "jobs": st["submitted"],
         "finished": st["finished"], "failed": st["failed"],
         "step_p50_ms": st["step_p50_ms"], "step_p99_ms": st["step_p99_ms"],
@@ -487,7 +508,7 @@ def bench_serve(pa, gen, args):
     [2,048, 262,144], steps uniform in [50, 500], FP64 or FP32c 50/50, 200 GB
     HBM budget."""
     r = serving_run(pa, gen, duration_s=args.serve_seconds, rate_hz=args.rate, n_lo=2048,
-                    n_hi=262144, s_lo=50, s_hi=500, drain_factor=args.drain)
+                    n_hi=262144, s_lo=50, s_hi=500, drain_factor=args.drain, check_failed=True)
     return {"metric": "serving: p50/p99 step and job latency, aggregate pair-interactions/s",
             "value": r["pairs_per_s"], "unit": "pair-interactions/s", "n_gpus": 1, "dtype": "mixed",
             "config": {"workload": "serving_stream", "n_range": [2048, 262144],

@@ -625,11 +625,20 @@ __device__ __forceinline__ void sym_steps(const SrcTile &tile, int s_begin, int 
     }
 }
 
+// Occupancy: the HIGHORDER kernel with one target per thread is held to 128
+// VGPRs, i.e. 4 waves per SIMD (the LDS of a 256-thread block, 29 KB, allows
+// 5 blocks per CU).  Left to itself the compiler takes 148 (3 waves).  The
+// unrolled steps are unchanged (the limit costs no spill inside them); at
+// N = 262,144 the pass takes 65.3 ms instead of 68.0.  Five waves (96 VGPRs)
+// spill inside the steps and take 90 ms; two targets per thread at three
+// waves (168 VGPRs) take 74.4 ms instead of 72.0; GAUSSIAN spills at 128.
+// Other instantiations keep the compiler's choice.
 template <int KIND, int TB, int TPT>
-__global__ __launch_bounds__(TB) void pair32s_kernel(const JobDev *__restrict__ jobs,
-                                                     const Item *__restrict__ items,
-                                                     const int *__restrict__ nitems_p,
-                                                     int *__restrict__ counter) {
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
+    (KIND == KIND_HIGHORDER && TPT == 1) ? 4 : 1))) void pair32s_kernel(const JobDev *__restrict__ jobs,
+                                                                        const Item *__restrict__ items,
+                                                                        const int *__restrict__ nitems_p,
+                                                                        int *__restrict__ counter) {
     constexpr int NW = TB / 64;
     constexpr int TW = TB * TPT;
     constexpr int SB = PW_SYM_RUN;         // steps per target-side binary32 run

@@ -50,13 +50,15 @@ def test_host_only_entry_points():
     assert lib.pw_job_footprint(262144, 0) > 262144 * 64
     # binary32 records carry binary64 strengths too: 80 B instead of 64 B
     # binary32: 16 more bytes per record, plus the symmetric pass's source-side
-    # slots, NCOMP_S = 10 binary64 sums per (particle, block of 256 x 2 targets) for
-    # the blocks at or before the particle's own
-    tw, npad = 512, 262144
-    nblk = npad // tw
-    slots = 10 * (nblk * npad - tw * nblk * (nblk - 1) // 2)
-    assert (lib.pw_job_footprint(262144, 1) - lib.pw_job_footprint(262144, 0)
-            == 16 * 262144 + 8 * slots)
+    # slots, NCOMP_S = 10 binary64 sums per (particle, block of 256 x TPT targets)
+    # for the blocks at or before the particle's own; TPT = 1 up to 393,216
+    # particles, 2 beyond
+    for n, tw in ((262144, 256), (393216, 256), (524288, 512)):
+        npad = n
+        nblk = npad // tw
+        slots = 10 * (nblk * npad - tw * nblk * (nblk - 1) // 2)
+        assert (lib.pw_job_footprint(n, 1) - lib.pw_job_footprint(n, 0)
+                == 16 * n + 8 * slots), n
 
 
 def test_null_arguments_rejected():
