@@ -225,7 +225,9 @@ int pw_batch_get(pw_batch *b, int64_t id, double *pos, double *vort, double *vel
 /* per-job status: PW_OK or PW_ENONFINITE after the last step */
 int pw_batch_status(pw_batch *b, int64_t id);
 int pw_batch_last_timing(const pw_batch *b, double *total_ms, double *pair_ms, double *pairs);
-/* source chunk length used for a job with n particles (a function of n only) */
+/* source chunk length of the one-sided pass for a job with n particles (a
+ * function of n and block only; jobs on the symmetric pass use 16-64 chunks,
+ * a function of n, block and targets per thread) */
 size_t pw_chunk_for(size_t n, int block);
 
 /* ---------------------------------------------------------------- serving

@@ -66,6 +66,15 @@ size_t chunk_for(size_t n, int TB) {
     return (size_t)TB * ((ntile + nchunk - 1) / nchunk);
 }
 
+size_t chunk_for_sym(size_t n, int TB, int TPT) {
+    if (n == 0) return (size_t)TB;
+    const size_t ntile = (n + TB - 1) / TB;
+    const size_t nblk = (n + (size_t)TB * TPT - 1) / ((size_t)TB * TPT);
+    size_t nchunk = std::min<size_t>(64, std::max<size_t>(16, (16384 + nblk - 1) / nblk));
+    nchunk = std::min(nchunk, ntile);
+    return (size_t)TB * ((ntile + nchunk - 1) / nchunk);
+}
+
 static size_t rec_bytes(int precision) {
     return precision == PREC_FP64 ? sizeof(Rec64) : sizeof(Rec32);
 }
@@ -108,7 +117,7 @@ size_t jpart_count(size_t n, size_t npad, size_t tw) {
 
 size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym) {
     size_t npad = round_up(std::max<size_t>(n, 1), 256);
-    size_t L = chunk_for(n, TB);
+    size_t L = sym ? chunk_for_sym(n, TB, TPT) : chunk_for(n, TB);
     size_t nchunk = (n + L - 1) / L;
     size_t b = npad * (rec_bytes(precision) + 5 * 6 * sizeof(double) + 2 * sizeof(double) +
                        nchunk * NCOMP * sizeof(double)) +
@@ -305,7 +314,9 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     j->npad = round_up(n, 256);
     j->par = par;
     j->kind = kind_;
-    size_t L = chunk_override ? chunk_override : chunk_for(n, tb_);
+    j->sym = par.precision == PREC_FP32C ? sym32_ != 0 : sym64_ != 0;
+    size_t L = chunk_override ? chunk_override
+                              : (j->sym ? chunk_for_sym(n, tb_, tpt_) : chunk_for(n, tb_));
     if (L % (size_t)tb_ != 0) throw Error(PW_EINVAL, "chunk must be a multiple of the block size");
     j->chunk_len = (int)L;
     j->nchunk = (int)((n + L - 1) / L);
@@ -325,7 +336,6 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     j->part.reset((size_t)j->nchunk * NCOMP * np * sizeof(double));
     j->vol.reset(2 * np * sizeof(double));  // volume | radius
     j->flag.reset(256);
-    j->sym = par.precision == PREC_FP32C ? sym32_ != 0 : sym64_ != 0;
     j->tw = tb_ * tpt_;
     alloc_jpart(j.get());
     j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + j->flag.bytes +
@@ -427,7 +437,8 @@ void Engine::append(Job *j, size_t m, const double *pos, const double *vort, con
                           hipMemcpyHostToDevice, stream_));
     PW_HIP(hipStreamSynchronize(stream_));
     j->n = n2;
-    const size_t L = j->chunk_override ? j->chunk_override : chunk_for(n2, tb_);
+    const size_t L = j->chunk_override ? j->chunk_override
+                                       : (j->sym ? chunk_for_sym(n2, tb_, tpt_) : chunk_for(n2, tb_));
     j->chunk_len = (int)L;
     j->nchunk = (int)((n2 + L - 1) / L);
     j->ntile = (int)((n2 + tb_ * tpt_ - 1) / (tb_ * tpt_));

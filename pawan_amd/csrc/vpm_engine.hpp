@@ -60,6 +60,12 @@ struct JobParams {
 // at most 16 chunks per job, chunk a multiple of TB.  Depends on (n, TB) only,
 // so a job's sums are formed the same way alone or in any batch.
 size_t chunk_for(size_t n, int TB);
+// Source chunk length for a job on the symmetric pass (blocks of TB x TPT
+// targets): enough chunks that the upper-triangle work items number at least
+// ~16,384 / 2 (8 per workgroup of a full persistent grid), so that the last
+// round of items is a small part of the launch; 16 to 64 chunks, a multiple
+// of TB.  Depends on (n, TB, TPT) only.
+size_t chunk_for_sym(size_t n, int TB, int TPT);
 
 // Bytes a resident job occupies on the device (TB x TPT targets per block;
 // sym: the job uses the symmetric pass and its source-side slots).
