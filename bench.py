@@ -640,6 +640,8 @@ def main():
             "speedup_vs_fp64_symmetric": (r32["pairs_per_s"] / line["fp64_symmetric"]["value"]
                                           if "fp64_symmetric" in line else None),
             "pair_kernel_ms": r32["pair_kernel_ms_mean"],
+            # per RHS, pair kernels only (the step adds the reduction and RK4 combine)
+            "pair_kernel_speedup_vs_fp64": r64["pair_kernel_ms_mean"] / r32["pair_kernel_ms_mean"],
             "scheme": "symmetric" if sym else "one-sided", "flop_per_pair": flop32,
             "frac_fp32_peak": flop32 * r32["pairs_per_launch"] /
             (r32["pair_kernel_ms_mean"] * 1e-3) / FP32_VECTOR_PEAK,
