@@ -209,7 +209,9 @@ typedef struct pw_job_spec {
 } pw_job_spec;
 
 /* variant may be NULL; zero fields take the defaults: block 256, two targets
- * per thread, default unroll, symmetric binary32 pass */
+ * per thread for binary64 jobs (one for GAUSSIAN) and one for binary32 jobs,
+ * default unroll, symmetric binary32 pass; a nonzero targets_per_thread
+ * applies to both precisions */
 int pw_batch_create(pw_batch **out, int device, int kernel, const pw_variant *variant);
 int pw_batch_destroy(pw_batch *b);
 const char *pw_batch_last_error(const pw_batch *b);

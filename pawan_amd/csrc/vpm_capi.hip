@@ -115,7 +115,9 @@ struct pw_ctx {
         pw::Variant v = choose_variant();
         if (writer) writer->quiesce();  // queued frames read the current device state
         pw::Variant cur = eng ? eng->variant() : pw::Variant{};
-        if (!eng || cur.block != v.block || cur.tpt != v.tpt || cur.unroll64 != v.unroll64 ||
+        const int cur32 = cur.tpt32 ? cur.tpt32 : cur.tpt, v32 = v.tpt32 ? v.tpt32 : v.tpt;
+        if (!eng || cur.block != v.block || cur.tpt != v.tpt || cur32 != v32 ||
+            cur.unroll64 != v.unroll64 ||
             cur.unroll32 != v.unroll32 || cur.sym32 != v.sym32 || cur.sym64 != v.sym64) {
             job = nullptr;
             eng.reset();
@@ -473,15 +475,20 @@ extern "C" {
 // Batch and server engines: 256 threads x 2 targets unless given (the best
 // binary64 variant at large N, and half the binary32 symmetric pass's slot
 // memory of one target per thread).
-static pw::Variant variant_of(const pw_variant *v, int precision_hint) {
+// Batches and servers: 256 threads; binary64 jobs two targets per thread
+// (the fastest one-sided HIGHORDER variant), binary32 jobs one (the fastest
+// symmetric pass); GAUSSIAN is compiled with one target per thread only.  An
+// explicit targets_per_thread applies to both.
+static pw::Variant variant_of(const pw_variant *v, int precision_hint, int kernel) {
     pw::Variant r;
     r.block = 256;
-    r.tpt = 2;
+    r.tpt = kernel == PW_KERNEL_HIGHORDER ? 2 : 1;
+    r.tpt32 = 1;
     if (!v) return r;
     if (v->scheme < PW_SCHEME_DEFAULT || v->scheme > PW_SCHEME_SYMMETRIC)
         throw Error(PW_EINVAL, "unknown pair scheme");
     r.block = v->block ? v->block : 256;
-    r.tpt = v->targets_per_thread ? v->targets_per_thread : 2;
+    if (v->targets_per_thread) r.tpt = r.tpt32 = v->targets_per_thread;
     r.sym32 = v->scheme != PW_SCHEME_ONESIDED;
     r.sym64 = v->scheme == PW_SCHEME_SYMMETRIC;
     if (v->unroll) {
@@ -500,7 +507,7 @@ int pw_batch_create(pw_batch **out, int device, int kernel, const pw_variant *va
     *out = nullptr;
     return guarded(nullptr, [&] {
         auto b = std::make_unique<pw_batch>();
-        b->eng.reset(new Engine(device, kernel, variant_of(variant, -1)));
+        b->eng.reset(new Engine(device, kernel, variant_of(variant, -1, kernel)));
         *out = b.release();
     });
 }
@@ -634,7 +641,7 @@ int pw_server_create(pw_server **out, int device, int kernel, size_t hbm_budget_
     *out = nullptr;
     return guarded(nullptr, [&] {
         auto s = std::make_unique<pw_server>();
-        s->eng.reset(new Engine(device, kernel, variant_of(variant, -1)));
+        s->eng.reset(new Engine(device, kernel, variant_of(variant, -1, kernel)));
         s->budget = hbm_budget_bytes;
         *out = s.release();
     });
@@ -664,7 +671,8 @@ int pw_server_submit(pw_server *s, const pw_job_spec *spec, size_t nsteps, doubl
         j.n = spec->n;
         j.nsteps = nsteps;
         const pw::Variant ev = s->eng->variant();
-        j.bytes = pw::job_footprint(spec->n, spec->precision, ev.block, ev.tpt,
+        const int tpt = spec->precision == PW_FP32C && ev.tpt32 ? ev.tpt32 : ev.tpt;
+        j.bytes = pw::job_footprint(spec->n, spec->precision, ev.block, tpt,
                                     spec->precision == PW_FP32C ? ev.sym32 != 0 : ev.sym64 != 0);
         if (j.bytes > s->budget) throw Error(PW_ENOMEM, "job larger than the HBM budget");
         j.arrival = arrival_ms < 0 ? s->now() : arrival_ms;

@@ -246,20 +246,23 @@ static PairFn pair_fn(int prec, int kind, int tb, int tpt, int unr) {
     return nullptr;
 }
 
-bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32, int sym32, int sym64) {
+bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32, int sym32, int sym64,
+                    int tpt32) {
+    const int t32 = tpt32 ? tpt32 : tpt;
     return (sym64 ? pair_fn_sym64(kind, tb, tpt) != nullptr
                   : pair_fn(PREC_FP64, kind, tb, tpt, unr64) != nullptr) &&
-           (sym32 ? pair_fn_sym(kind, tb, tpt) != nullptr
-                  : pair_fn(PREC_FP32C, kind, tb, tpt, unr32) != nullptr);
+           (sym32 ? pair_fn_sym(kind, tb, t32) != nullptr
+                  : pair_fn(PREC_FP32C, kind, tb, t32, unr32) != nullptr);
 }
 
 Engine::Engine(int device, int kind, const Variant &v)
     : device_(device), kind_(kind), tb_(v.block), tpt_(v.tpt), unr64_(v.unroll64),
-      unr32_(v.unroll32), sym32_(v.sym32 ? 1 : 0), sym64_(v.sym64 ? 1 : 0) {
+      unr32_(v.unroll32), sym32_(v.sym32 ? 1 : 0), sym64_(v.sym64 ? 1 : 0),
+      tpt32_(v.tpt32 ? v.tpt32 : v.tpt) {
     if (tb_ != 64 && tb_ != 128 && tb_ != 256) throw Error(PW_EINVAL, "block must be 64, 128 or 256");
     if (kind_ != KIND_HIGHORDER && kind_ != KIND_GAUSSIAN) throw Error(PW_EINVAL, "unknown kernel");
     fn64_ = sym64_ ? pair_fn_sym64(kind_, tb_, tpt_) : pair_fn(PREC_FP64, kind_, tb_, tpt_, unr64_);
-    fn32_ = sym32_ ? pair_fn_sym(kind_, tb_, tpt_) : pair_fn(PREC_FP32C, kind_, tb_, tpt_, unr32_);
+    fn32_ = sym32_ ? pair_fn_sym(kind_, tb_, tpt32_) : pair_fn(PREC_FP32C, kind_, tb_, tpt32_, unr32_);
     if (!fn64_ || !fn32_) throw Error(PW_EINVAL, "no compiled kernel variant for these parameters");
     int ndev = 0;
     PW_HIP(hipGetDeviceCount(&ndev));
@@ -315,12 +318,13 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     j->par = par;
     j->kind = kind_;
     j->sym = par.precision == PREC_FP32C ? sym32_ != 0 : sym64_ != 0;
+    const int tpt = tpt_of(par.precision);
     size_t L = chunk_override ? chunk_override
-                              : (j->sym ? chunk_for_sym(n, tb_, tpt_) : chunk_for(n, tb_));
+                              : (j->sym ? chunk_for_sym(n, tb_, tpt) : chunk_for(n, tb_));
     if (L % (size_t)tb_ != 0) throw Error(PW_EINVAL, "chunk must be a multiple of the block size");
     j->chunk_len = (int)L;
     j->nchunk = (int)((n + L - 1) / L);
-    j->ntile = (int)((n + tb_ * tpt_ - 1) / (tb_ * tpt_));
+    j->ntile = (int)((n + tb_ * tpt - 1) / (tb_ * tpt));
     j->chunk_override = chunk_override;
     j->part_chunks = j->nchunk;
     j->uniform = true;
@@ -336,7 +340,7 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     j->part.reset((size_t)j->nchunk * NCOMP * np * sizeof(double));
     j->vol.reset(2 * np * sizeof(double));  // volume | radius
     j->flag.reset(256);
-    j->tw = tb_ * tpt_;
+    j->tw = tb_ * tpt;
     alloc_jpart(j.get());
     j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + j->flag.bytes +
                j->jpart.bytes;
@@ -437,11 +441,12 @@ void Engine::append(Job *j, size_t m, const double *pos, const double *vort, con
                           hipMemcpyHostToDevice, stream_));
     PW_HIP(hipStreamSynchronize(stream_));
     j->n = n2;
+    const int tpt = tpt_of(j->par.precision);
     const size_t L = j->chunk_override ? j->chunk_override
-                                       : (j->sym ? chunk_for_sym(n2, tb_, tpt_) : chunk_for(n2, tb_));
+                                       : (j->sym ? chunk_for_sym(n2, tb_, tpt) : chunk_for(n2, tb_));
     j->chunk_len = (int)L;
     j->nchunk = (int)((n2 + L - 1) / L);
-    j->ntile = (int)((n2 + tb_ * tpt_ - 1) / (tb_ * tpt_));
+    j->ntile = (int)((n2 + tb_ * tpt - 1) / (tb_ * tpt));
     if (j->nchunk > j->part_chunks) {
         j->part_chunks = std::max(j->nchunk, 16);
         j->part.reset((size_t)j->part_chunks * NCOMP * np * sizeof(double));

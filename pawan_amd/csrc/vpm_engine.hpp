@@ -106,9 +106,10 @@ struct Variant {
     int block = 128, tpt = 1, unroll64 = 2, unroll32 = 4;
     int sym32 = 1;  // binary32 jobs: 1 symmetric pass, 0 one-sided pass
     int sym64 = 0;  // binary64 jobs: 1 symmetric pass, 0 one-sided pass (the plain method)
+    int tpt32 = 0;  // targets per thread of binary32 jobs; 0: tpt
 };
 bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32, int sym32 = 1,
-                    int sym64 = 0);
+                    int sym64 = 0, int tpt32 = 0);
 
 class Engine {
   public:
@@ -120,7 +121,7 @@ class Engine {
     int device() const { return device_; }
     int kind() const { return kind_; }
     int block() const { return tb_; }
-    Variant variant() const { return Variant{tb_, tpt_, unr64_, unr32_, sym32_, sym64_}; }
+    Variant variant() const { return Variant{tb_, tpt_, unr64_, unr32_, sym32_, sym64_, tpt32_}; }
     hipStream_t stream() const { return stream_; }
 
     // Create a job from N x 3 positions/vorticities and N radius/volume.
@@ -176,6 +177,8 @@ class Engine {
     double pairs_per_rhs() const;
 
     int device_, kind_, tb_, tpt_, unr64_, unr32_, sym32_, sym64_;
+    int tpt32_;  // targets per thread of binary32 jobs (tpt_: binary64 jobs)
+    int tpt_of(int precision) const { return precision == PREC_FP32C ? tpt32_ : tpt_; }
     void alloc_jpart(Job *j);
     void (*fn64_)(const JobDev *, const Item *, const int *, int *) = nullptr;
     void (*fn32_)(const JobDev *, const Item *, const int *, int *) = nullptr;
