@@ -120,3 +120,24 @@ def test_server_budget():
 def _args(job, steps):
     w, nu, dt, prec, uinf = job
     return (w, nu, dt, steps, prec, uinf)
+
+
+def test_gaussian_batch_default_variant():
+    """A GAUSSIAN batch with the default variant (one target per thread for
+    both precisions: the GAUSSIAN kernels are compiled so) matches the oracle."""
+    jobs = _jobs()[:2]
+    print("case GAUSSIAN batch, default variant", flush=True)
+    B = pa.Batch(kernel=pa.KERNEL_GAUSSIAN)
+    ids = [B.add(w, nu, dt, prec, uinf) for (w, nu, dt, prec, uinf) in jobs]
+    B.step(2, graph=True)
+    out = [B.get(i) for i in ids]
+    B.close()
+    for k, (w, nu, dt, prec, uinf) in enumerate(jobs):
+        x0 = orc.pack_state(w["pos"], w["vort"])
+        xo = orc.rk4(x0, w["radius"], w["volume"], nu, dt, 2, mode=orc.OMP_CLEAN, kind=orc.GAUSSIAN,
+                     uinf=np.array(uinf, dtype=float))
+        po, vo = orc.unpack_state(xo)
+        ep, ea = per_particle_rel(out[k]["pos"], po), per_particle_rel(out[k]["vort"], vo)
+        print(f"  job {k} prec {prec}: pos {ep:.3e} vort {ea:.3e}", flush=True)
+        assert ep < (1e-11 if prec == pa.FP64 else 1e-6)
+        assert ea < (1e-9 if prec == pa.FP64 else 1e-5)

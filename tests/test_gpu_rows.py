@@ -158,7 +158,8 @@ def test_emission_into_first_of_two_wakes():
 def test_async_frames_equal_sync_frames(tmp_path, precision):
     """Frames queued with write_async (snapshot on the device, copy stream,
     writer thread) are byte-identical to synchronous write() frames of the
-    same run, including frames after the wake grows by emission."""
+    same run, including frames after emission into the first wake (a
+    rebuild of the device arrays while frames are queued) and into the last."""
     a, b, _ = _two_wakes()
     e = gen.vortex_ring(8, 16, seed=34, center=(0.0, 0.0, -0.4))
     print(f"case async frames precision={precision}", flush=True)
@@ -169,7 +170,10 @@ def test_async_frames_equal_sync_frames(tmp_path, precision):
     with pa.WakeFile(pa_) as f:
         S2.write_async(f, 0.0)
         for k in range(1, 7):
-            if k == 4:
+            if k == 2:  # into the first wake: the device arrays are rebuilt
+                for S in (S1, S2):
+                    S.appendParticles(0, e["pos"], e["vort"], e["radius"], e["volume"])
+            if k == 4:  # into the last wake: appended in place
                 for S in (S1, S2):
                     S.appendParticles(1, e["pos"], e["vort"], e["radius"], e["volume"])
             S1.rk4(0.01, 1)
