@@ -11,9 +11,9 @@
 //   Interaction  virtual addWake(Wake*), solve(), resolve(), write(FILE*), diagnose(),
 //                setStates(const Vector*), getRates(Vector*), getStates(Vector*)
 //                (interaction.h:181-234); protected double _nu (interaction.h:24)
-//   Wake         _numParticles; _position, _vorticity, _velocity, _retvorcity
-//                and _vorticityfield with ->data holding N x 3 row-major
-//                doubles (tda == 3);
+//   Wake         _size, _numParticles; _position, _vorticity, _velocity,
+//                _retvorcity and _vorticityfield with ->data holding N x 3
+//                row-major doubles (tda == 3);
 //                _radius, _volume with ->data holding N doubles (wake.h:33-42)
 //   Vector       ->data holding the state/rate vector (gsl_vector, stride 1)
 //   RK4          RK4(const double &t, const size_t &n); protected virtual
@@ -21,7 +21,13 @@
 //
 // Semantics follow the reference: solve() fills every wake's _velocity and
 // _retvorcity; setStates() also updates the wakes' _position/_vorticity;
-// write() and diagnose() see the device state.  RK4 steps run entirely on
+// write() and diagnose() see the device state.  A wake whose state is not
+// [positions 3N | vorticities 3N] (_size != 6 _numParticles: __vring, whose
+// state is its cross-section strengths, vring.cpp:55,110-128) keeps its own
+// state and rates: setStates/getStates/getRates then go through the wakes'
+// own virtuals (interaction.cpp:204-229) and the device is given the
+// particles' positions and vorticities those produce, so __resolve::rebuild
+// (resolve.cpp:10-57) runs on the device-backed interaction unchanged.  RK4 steps run entirely on
 // the device (pw_rk4_steps); the integrator's state vector is refreshed after
 // each step, as __rk4::step leaves it.
 #pragma once
@@ -104,11 +110,34 @@ class interaction : public Interaction, public device_system {
     // interaction.cpp:204-211
     void setStates(const Vector *state) override {
         Interaction::setStates(state);
-        check(pw_set_state(ctx_, state->data), ctx_);
+        if (plain()) {
+            check(pw_set_state(ctx_, state->data), ctx_);
+            return;
+        }
+        // the wakes' particles as the wakes' own setStates left them
+        std::vector<double> st;
+        for (Wake *W : wakes_) {
+            const size_t m = 3 * W->_numParticles;
+            st.insert(st.end(), W->_position->data, W->_position->data + m);
+            st.insert(st.end(), W->_vorticity->data, W->_vorticity->data + m);
+        }
+        check(pw_set_state(ctx_, st.data()), ctx_);
     }
     // interaction.cpp:213-229
-    void getRates(Vector *rate) override { check(pw_get_rates(ctx_, rate->data), ctx_); }
-    void getStates(Vector *state) override { check(pw_get_state(ctx_, state->data), ctx_); }
+    void getRates(Vector *rate) override {
+        if (plain())
+            check(pw_get_rates(ctx_, rate->data), ctx_);
+        else
+            Interaction::getRates(rate);  // from the rates solve()/resolve() left in the wakes
+    }
+    void getStates(Vector *state) override {
+        if (plain()) {
+            check(pw_get_state(ctx_, state->data), ctx_);
+        } else {
+            pull();
+            Interaction::getStates(state);
+        }
+    }
 
     // interaction.cpp:45-47 -> influence(): every wake's _vorticityfield
     void resolve() override {
@@ -150,6 +179,12 @@ class interaction : public Interaction, public device_system {
     }
 
   private:
+    // every wake's state is its particles' [positions | vorticities]
+    bool plain() const {
+        for (Wake *W : wakes_)
+            if (W->_size != 6 * W->_numParticles) return false;
+        return true;
+    }
     void pull_rates() {
         for (size_t k = 0; k < wakes_.size(); ++k) {
             Wake *W = wakes_[k];
@@ -200,7 +235,7 @@ class rk4 : public RK4 {
         device_system *d = dynamic_cast<device_system *>(S);
         if (!d) throw std::invalid_argument("pawan_gpu::rk4 steps a pawan_gpu::interaction");
         check(pw_rk4_steps(d->context(), dt, 1), d->context());
-        check(pw_get_state(d->context(), states->data), d->context());
+        S->getStates(states);
     }
 };
 
@@ -219,7 +254,7 @@ class euler : public Integration {
         device_system *d = dynamic_cast<device_system *>(S);
         if (!d) throw std::invalid_argument("pawan_gpu::euler steps a pawan_gpu::interaction");
         check(pw_euler_steps(d->context(), dt, 1), d->context());
-        check(pw_get_state(d->context(), states->data), d->context());
+        S->getStates(states);
     }
 };
 

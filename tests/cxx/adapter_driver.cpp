@@ -158,6 +158,56 @@ int main(int argc, char **argv) {
             delete A1;
             delete A2;
         }
+        {  // __resolve::rebuild's sequence (resolve.cpp:18-31) on a wake whose state is one
+           // strength per particle: column i of the influence matrix is getRates() after
+           // setStates(e_i) and resolve()
+            const size_t m = 48;
+            auto *R = new mini::strength_wake(m);
+            for (size_t i = 0; i < m; ++i) {
+                const double psi = 2.0 * M_PI * (double)i / (double)m;
+                R->_position->data[3 * i] = std::cos(psi);
+                R->_position->data[3 * i + 1] = std::sin(psi);
+                R->_position->data[3 * i + 2] = 0.02 * std::sin(3.0 * psi);
+                R->dir[3 * i] = -std::sin(psi);
+                R->dir[3 * i + 1] = std::cos(psi);
+                R->dir[3 * i + 2] = 0.0;
+                R->_radius->data[i] = 0.15;
+                R->_volume->data[i] = 1e-3;
+            }
+            std::vector<double> A(m * m), Aref(m * m);
+            {
+                gpu_interaction SR(R, o);
+                mini::vec st(SR._size), col(SR._size);
+                for (size_t i = 0; i < m; ++i) {
+                    for (size_t k = 0; k < m; ++k) st.data[k] = k == i ? 1.0 : 0.0;
+                    SR.setStates(&st);
+                    SR.resolve();
+                    SR.getRates(&col);
+                    for (size_t k = 0; k < m; ++k) A[k * m + i] = col.data[k];
+                }
+                SR.getStates(&st);
+                bool sst = true;
+                for (size_t k = 0; k < m; ++k) sst = sst && st.data[k] == (k == m - 1 ? 1.0 : 0.0);
+                std::printf("driver: strength-wake getStates %s\n", sst ? "exact" : "WRONG");
+                bad |= !sst;
+            }
+            std::vector<double> rp(R->_position->data, R->_position->data + 3 * m), rv(3 * m),
+                rr(R->_radius->data, R->_radius->data + m), fl(3 * m);
+            size_t wm[1] = {m};
+            for (size_t i = 0; i < m; ++i) {
+                std::fill(rv.begin(), rv.end(), 0.0);
+                for (int c = 0; c < 3; ++c) rv[3 * i + c] = R->dir[3 * i + c];
+                orc_influence(ORC_KERNEL_HIGHORDER, 1, wm, rp.data(), rv.data(), rr.data(), fl.data());
+                for (size_t k = 0; k < m; ++k)
+                    Aref[k * m + i] = std::sqrt(fl[3 * k] * fl[3 * k] + fl[3 * k + 1] * fl[3 * k + 1] +
+                                                fl[3 * k + 2] * fl[3 * k + 2]);
+            }
+            const double ea = relerr(A, Aref);
+            std::printf("driver: rebuild influence matrix (strength wake, %zu x %zu) vs oracle: %.3e\n",
+                        m, m, ea);
+            bad |= !(ea < 1e-12);
+            delete R;
+        }
         double om[3], im[3], ke;
         S.device_diagnostics(om, im, &ke);
         std::printf("driver: kinetic energy %.12e\n", ke);

@@ -7,6 +7,7 @@
 // .wake frame format (interaction.cpp:197-202, wake.cpp:68-73) and the
 // integrate() loop (integration.cpp:16-35).  Its own CPU solve() is unused.
 #pragma once
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -56,9 +57,42 @@ class wake {
         fwrite(_vorticity->data, sizeof(double), 3 * _numParticles, f);
         fwrite(_radius->data, sizeof(double), _numParticles, f);
     }
-    void setStates(const double *s) {  // wake.cpp:87-96
+    virtual void setStates(const double *s) {  // wake.cpp:87-96
         std::memcpy(_position->data, s, 3 * _numParticles * sizeof(double));
         std::memcpy(_vorticity->data, s + 3 * _numParticles, 3 * _numParticles * sizeof(double));
+    }
+    virtual void getStates(double *s) const {  // wake.cpp:109-120
+        std::memcpy(s, _position->data, 3 * _numParticles * sizeof(double));
+        std::memcpy(s + 3 * _numParticles, _vorticity->data, 3 * _numParticles * sizeof(double));
+    }
+    virtual void getRates(double *r) const {  // wake.cpp:98-107
+        std::memcpy(r, _velocity->data, 3 * _numParticles * sizeof(double));
+        std::memcpy(r + 3 * _numParticles, _retvorcity->data, 3 * _numParticles * sizeof(double));
+    }
+};
+
+// A wake whose state is not its particles' positions and vorticities, as
+// __vring's (vring.cpp:55,110-128): one strength per particle along a fixed
+// unit direction; its rates are the magnitudes of the vorticity field that
+// resolve() leaves in _vorticityfield.  _size = N, not 6 N.
+class strength_wake : public wake {
+  public:
+    std::vector<double> dir, strength;
+    explicit strength_wake(size_t n) : wake(n), dir(3 * n), strength(n) { _size = n; }
+    void setStates(const double *s) override {
+        for (size_t i = 0; i < _numParticles; ++i) {
+            strength[i] = s[i];
+            for (int c = 0; c < 3; ++c) _vorticity->data[3 * i + c] = s[i] * dir[3 * i + c];
+        }
+    }
+    void getStates(double *s) const override {
+        for (size_t i = 0; i < _numParticles; ++i) s[i] = strength[i];
+    }
+    void getRates(double *r) const override {
+        for (size_t i = 0; i < _numParticles; ++i) {
+            const double *f = _vorticityfield->data + 3 * i;
+            r[i] = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        }
     }
 };
 
@@ -83,10 +117,25 @@ class interaction : public system {
         fwrite(&_nWake, sizeof(size_t), 1, f);
         for (auto *w : _W) w->write(f);
     }
+    // interaction.cpp:204-229: each wake reads or writes its own _size slice
     void setStates(const vec *s) override {
         size_t off = 0;
         for (auto *w : _W) {
             w->setStates(s->data + off);
+            off += w->_size;
+        }
+    }
+    void getStates(vec *s) override {
+        size_t off = 0;
+        for (auto *w : _W) {
+            w->getStates(s->data + off);
+            off += w->_size;
+        }
+    }
+    void getRates(vec *r) override {
+        size_t off = 0;
+        for (auto *w : _W) {
+            w->getRates(r->data + off);
             off += w->_size;
         }
     }
