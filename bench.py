@@ -118,6 +118,18 @@ def rates_split_error(S64, S32, w):
             "vorticity_rate": rr(a[3], b[3])}
 
 
+def split_error_for(pa, w, nu, uinf=(0.0, 0.0, 0.0)):
+    """rates_split_error for the particles of w (one wake) at their state."""
+    ww = {k: w[k] for k in ("pos", "vort", "radius", "volume")}
+    ww["n"], ww["nu"] = int(w["pos"].shape[0]), nu
+    S64 = pa.Interaction(pa.Wake.from_dict(ww), nu=nu, uinf=uinf)
+    S32 = pa.Interaction(pa.Wake.from_dict(ww), nu=nu, uinf=uinf, precision=pa.FP32C)
+    e = rates_split_error(S64, S32, ww)
+    S64.close()
+    S32.close()
+    return e
+
+
 def cpu_baseline(w, budget_s=15.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc  # CPU baseline leg only
@@ -222,6 +234,10 @@ def serving_run(pa, gen, duration_s=20.0, rate_hz=4.0, seed=1, budget=200 << 30,
         "step_p50_ms": st["step_p50_ms"], "step_p99_ms": st["step_p99_ms"],
         "job_p50_ms": st["job_p50_ms"], "job_p99_ms": st["job_p99_ms"],
         "pairs_per_s": st["pairs"] / max(st["busy_ms"], 1e-9) * 1000.0,
+        # the served mix at the binary64 method's 95 algorithmic FLOP per ordered
+        # pair against the FP64 vector peak (binary32 jobs count at that rate too)
+        "frac_fp64_peak_95flop": (st["pairs"] / max(st["busy_ms"], 1e-9) * 1000.0 *
+                                  FLOP_PER_PAIR / FP64_VECTOR_PEAK),
         "steps": st["steps"], "n_range": [n_lo, n_hi], "steps_range": [s_lo, s_hi],
         "still_running": st["running"] + st["queued"],
     }
@@ -284,6 +300,7 @@ def bench_ring1(pa, gen, args):
             "steps": w["steps"], "ms_per_step": gpu_s * 1e3 / w["steps"], "dtype": "f64",
             "config": {"workload": "single_ring_2048", "dt": w["dt"], "nu": w["nu"]},
             "trajectory_max_rel_diff_vs_cpu": err, "drift": _drift(d0, d1),
+            "fp32c_rel_rms_err_vs_fp64_t0": split_error_for(pa, w, w["nu"]),
             "cpu_baseline": {"value": pairs / cpu_s, "seconds": cpu_s, "cores": orc.max_threads(),
                              "kind": "port", "sample": "the full 100-step run"}}
 
@@ -394,6 +411,7 @@ def bench_leapfrog(pa, gen, args):
     v = np.r_[r64[:3 * n1], r64[6 * n1:6 * n1 + 3 * (n - n1)]]
     v32 = np.r_[r32[:3 * n1], r32[6 * n1:6 * n1 + 3 * (n - n1)]]
     out["fp32c"]["rel_rms_rate_err_vs_fp64_t0"] = {"all": rr(r32, r64), "velocity": rr(v32, v)}
+    out["fp32c"]["rel_rms_err_vs_fp64_t0_per_term"] = split_error_for(pa, w, w["nu"])
     out["note"] = ("the reference HIGHORDER kernel has q ~ 1/rho^3 at small separation (the phi term "
                    "keeps t^-1/2 while Z decays as (t+1)^-7.5); over long runs the two rings' "
                    "particles meet and the state diverges in binary64 as in binary32")
@@ -437,12 +455,19 @@ def bench_rotor_growth(pa, gen, args):
             log(f"  rotor growth step {k} N={n} t={time.perf_counter() - t0:.1f} s")
     sec = time.perf_counter() - t0
     d = S.diagnose(ke=False)
+    # per-term error of the binary32 pairs at the grown state
+    st = S.getStates()
+    W = S._W[0]
+    fin = {"pos": st[:3 * n].reshape(n, 3), "vort": st[3 * n:].reshape(n, 3),
+           "radius": W._radius, "volume": W._volume}
     S.close()
+    err = split_error_for(pa, fin, 1e-4, uinf=(0.0, 0.0, -0.05))
     return {"metric": "rotor wake grown to N particles, FP32c pairs, FP64 state: pair-interactions/s",
             "value": pairs / sec, "unit": "pair-interactions/s", "n_gpus": 1, "steps": shed,
             "dtype": "f32c", "config": {"workload": f"rotor_growth_{n}", "final_n": n,
                                         "dt": LONG_DT}, "seconds": sec,
-            "total_vorticity": d["total_vorticity"].tolist()}
+            "total_vorticity": d["total_vorticity"].tolist(),
+            "fp32c_rel_rms_err_vs_fp64_end": err}
 
 
 def bench_batch(pa, gen, args):
@@ -497,6 +522,13 @@ def bench_batch(pa, gen, args):
     out["value"] = out["fp64"]["value"]
     out["dtype"] = "f64"
     out["fp32c"]["speedup_vs_fp64"] = out["fp32c"]["value"] / out["fp64"]["value"]
+    # per-term binary32 errors at t = 0 on the 4 largest and 4 other jobs
+    order = sorted(range(len(jobs)), key=lambda k: -jobs[k]["n"])
+    sample = order[:4] + order[4::max(1, (len(order) - 4) // 4)][:4]
+    errs = [split_error_for(pa, jobs[k], jobs[k]["nu"]) for k in sample]
+    out["fp32c"]["rel_rms_err_vs_fp64_t0_sample"] = {
+        "jobs": sample, "max": {t: max(e[t] for e in errs) for t in errs[0]},
+        "median": {t: float(np.median([e[t] for e in errs])) for t in errs[0]}}
     # jobs the binary32 path flags that did not also diverge in binary64
     out["fp32c"]["failed_not_diverged_in_fp64"] = len(sets["fp32c"][0] - sets["fp64"][1])
     return out

@@ -52,13 +52,16 @@ def test_host_only_entry_points():
     # binary32: 16 more bytes per record, plus the symmetric pass's source-side
     # slots, NCOMP_S = 10 binary64 sums per (particle, block of 256 x TPT targets)
     # for the blocks at or before the particle's own; TPT = 1 up to 393,216
-    # particles, 2 beyond
-    for n, tw in ((262144, 256), (393216, 256), (524288, 512)):
+    # particles, 2 beyond.  The symmetric pass also takes 16-64 source chunks
+    # (enough that its triangle of work items is ~8,192 long) where the
+    # one-sided pass takes 16: 13 binary64 partial sums per chunk and particle.
+    for n, tw, nch in ((262144, 256, 16), (393216, 256, 16), (524288, 512, 16), (65536, 256, 64),
+                       (131072, 256, 32)):
         npad = n
         nblk = npad // tw
         slots = 10 * (nblk * npad - tw * nblk * (nblk - 1) // 2)
         assert (lib.pw_job_footprint(n, 1) - lib.pw_job_footprint(n, 0)
-                == 16 * n + 8 * slots), n
+                == 16 * n + 8 * slots + (nch - 16) * 13 * 8 * npad), n
 
 
 def test_null_arguments_rejected():
