@@ -811,9 +811,12 @@ __device__ __forceinline__ double rot64(double x) {
     return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// The binary64 tile, laid out as SrcTile for the same reason.
+// The binary64 tile, laid out as SrcTile for the same reason; v and h stay
+// separate arrays (a uniform-core job reads v alone: half of a 16-byte pair
+// would be read at a 16-byte lane stride, with 2-way bank conflicts).
 struct SrcTile64 {
-    double2 xy[128], za[128], aa[128], vh[128];  // (x, y), (z, ax), (ay, az), (v, h)
+    double2 xy[128], za[128], aa[128];  // (x, y), (z, ax), (ay, az)
+    double v[128], h[128];
 };
 
 struct Part64 {
@@ -821,8 +824,8 @@ struct Part64 {
 };
 
 __device__ __forceinline__ Part64 load_src64(const SrcTile64 &t, int k) {
-    const double2 a = t.xy[k], b = t.za[k], c = t.aa[k], d = t.vh[k];
-    return Part64{a.x, a.y, b.x, b.y, c.x, c.y, d.x, d.y};
+    const double2 a = t.xy[k], b = t.za[k], c = t.aa[k];
+    return Part64{a.x, a.y, b.x, b.y, c.x, c.y, t.v[k], t.h[k]};
 }
 
 // One unordered pair in binary64: target-side terms added to a, source-side
@@ -977,7 +980,8 @@ __global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ 
                     tile.xy[k + c] = make_double2(r.x, r.y);
                     tile.za[k + c] = make_double2(r.z, r.ax);
                     tile.aa[k + c] = make_double2(r.ay, r.az);
-                    tile.vh[k + c] = make_double2(r.v, r.h);
+                    tile.v[k + c] = r.v;
+                    tile.h[k + c] = r.h;
                 }
             }
             __syncthreads();
