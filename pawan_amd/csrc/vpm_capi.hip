@@ -51,13 +51,13 @@ int default_block(size_t n) {
     return 64;
 }
 
-// One target per thread is the faster binary32 symmetric pass (65.3 against
-// 72.0 ms at N = 262,144), but its source-side slots take 40 N^2 / (block x
-// targets per thread) bytes: past N = 393,216 (24 GB at 256 threads) two
-// targets per thread halve them.  (The GAUSSIAN kernel is compiled with one
-// target per thread only.)
+// One target per thread is the faster binary32 symmetric pass (59.9 against
+// 68.2 ms at N = 262,144; 244 against 270 ms at 524,288), but its source-side
+// slots take 40 N^2 / (block x targets per thread) bytes: past N = 524,288
+// (43 GB at 256 threads) two targets per thread halve them.  (The GAUSSIAN
+// kernel is compiled with one target per thread only.)
 int default_tpt(size_t n, int precision, int kernel) {
-    return kernel == PW_KERNEL_HIGHORDER && precision == PW_FP32C && n > 393216 ? 2 : 1;
+    return kernel == PW_KERNEL_HIGHORDER && precision == PW_FP32C && n > 524288 ? 2 : 1;
 }
 
 bool finite_params(double nu, double dt, const double *u) {
