@@ -446,7 +446,7 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
 // saves about a third of the arithmetic per ordered pair.  A workgroup owns a
 // block of TB x TPT targets (thread t holds targets t, t + TB, ...) and walks
 // the sources j > i of one chunk, 64 at a time, staged through LDS as
-// structure of arrays.  Within a 64-source tile, lane l takes source
+// arrays of 16-byte field pairs.  Within a 64-source tile, lane l takes source
 // (l + s) mod 64 at step s, so the 64 lanes meet 64 different sources at
 // every step and each source meets every lane once in 64 steps.  The source-
 // side sums ride along: each step they move one lane down (DPP wave rotate,
