@@ -255,13 +255,19 @@ class WakeFile:
 
     def close(self):
         if self.handle:
-            try:
-                for S in self.users:
-                    if getattr(S, "_h", None):
+            # every writer of this file drains before the FILE is closed; the
+            # first failure is raised afterwards
+            err = None
+            for S in self.users:
+                if getattr(S, "_h", None):
+                    try:
                         S.write_wait()
-            finally:
-                self._libc.fclose(self.handle)
-                self.handle = None
+                    except Exception as e:  # noqa: BLE001
+                        err = err or e
+            self._libc.fclose(self.handle)
+            self.handle = None
+            if err is not None:
+                raise err
 
     def __enter__(self):
         return self
