@@ -230,7 +230,11 @@ int pw_add_wake(pw_ctx *ctx, size_t n, const double *pos, const double *vort,
     return guarded(&ctx->err, [&] {
         if (n == 0 || !pos || !vort || !radius || !volume)
             throw Error(PW_EINVAL, "empty wake or null array");
-        if (ctx->job) {  // keep the device state of the wakes already present
+        // keep the device state of the wakes already present; once the host
+        // arrays are ahead of the device (a wake added, particles inserted),
+        // they hold that state already and the device arrays no longer match
+        // their layout
+        if (ctx->job && !ctx->dirty) {
             ctx->eng->download_aos(ctx->job, 0, ctx->pos.data());
             ctx->eng->download_aos(ctx->job, 1, ctx->vort.data());
         }
@@ -472,9 +476,6 @@ static JobParams params_of(const pw_job_spec *s) {
 
 extern "C" {
 
-// Batch and server engines: 256 threads x 2 targets unless given (the best
-// binary64 variant at large N, and half the binary32 symmetric pass's slot
-// memory of one target per thread).
 // Batches and servers: 256 threads; binary64 jobs two targets per thread
 // (the fastest one-sided HIGHORDER variant), binary32 jobs one (the fastest
 // symmetric pass); GAUSSIAN is compiled with one target per thread only.  An

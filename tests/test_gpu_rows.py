@@ -154,6 +154,28 @@ def test_emission_into_first_of_two_wakes():
     assert np.max(np.abs(r - ref)) <= 1e-10 * np.max(np.abs(ref))
 
 
+def test_add_wake_after_emission_into_first_wake():
+    """A wake added while emission into the first wake is still only on the
+    host (no solve in between) keeps the emitted particles and the stepped
+    state of the others."""
+    a = gen.vortex_ring(16, 16, seed=41)
+    b = gen.vortex_ring(8, 9, seed=42, center=(0, 0, 0.5))
+    e = gen.vortex_ring(4, 16, seed=43, center=(0, 0, -0.3))
+    c = gen.vortex_ring(8, 8, seed=44, center=(0, 0, 1.0))
+    S = pa.Interaction(pa.Wake.from_dict(a), pa.Wake.from_dict(b), nu=1e-3)
+    S.rk4(0.005, 1)
+    stepped = S.getStates()
+    S.appendParticles(0, e["pos"], e["vort"], e["radius"], e["volume"])
+    S.addWake(pa.Wake.from_dict(c))
+    st = S.getStates()
+    S.close()
+    pa0, va0 = orc.unpack_state(stepped, [a["n"], b["n"]])
+    want_p = np.concatenate([pa0[: a["n"]], e["pos"], pa0[a["n"]:], c["pos"]])
+    want_v = np.concatenate([va0[: a["n"]], e["vort"], va0[a["n"]:], c["vort"]])
+    ns = [a["n"] + e["n"], b["n"], c["n"]]
+    assert np.array_equal(st, orc.pack_state(want_p, want_v, ns))
+
+
 @pytest.mark.parametrize("precision", [pa.FP64, pa.FP32C])
 def test_async_frames_equal_sync_frames(tmp_path, precision):
     """Frames queued with write_async (snapshot on the device, copy stream,
