@@ -1075,116 +1075,178 @@ __device__ __forceinline__ void store_rec<Rec32>(Rec32 *r, int i, const double *
     r[i].fz = (float)xs[5];
 }
 
+// Global-memory view of a binary64 array (lets the compiler address it as a
+// wave-uniform base plus a per-lane offset).
+typedef __attribute__((address_space(1))) const double gdouble;
+
+// Rates k of particle i: written out, checked, then the stage combination
+// and the next stage's record.
+__device__ __forceinline__ void finish_particle(const JobDev &J, int i, const double *k, int mode) {
+#pragma clang fp contract(off)
+    const size_t np = (size_t)J.npad;
+    bool finite = true;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        J.rate[m * np + i] = k[m];
+        finite = finite && __builtin_isfinite(k[m]);
+    }
+    if (!finite) atomicOr(J.flag, 1);
+    if (mode == MODE_SOLVE) return;
+    const double dt = J.dt;
+    double nx[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        const size_t o = m * np + i;
+        const double x0 = J.x0[o];
+        if (mode == MODE_RK4_1) {
+            J.k1[o] = k[m];
+            nx[m] = k[m] * (0.5 * dt) + x0;
+        } else if (mode == MODE_RK4_2) {
+            J.k23[o] = k[m];
+            nx[m] = k[m] * (0.5 * dt) + x0;
+        } else if (mode == MODE_RK4_3) {
+            J.k23[o] = J.k23[o] + k[m];
+            nx[m] = k[m] * dt + x0;
+        } else if (mode == MODE_RK4_4) {
+            const double a = (J.k1[o] + k[m]) * (dt / 6.);
+            const double b = J.k23[o] * (dt / 3.);
+            nx[m] = x0 + (a + b);
+            J.x0[o] = nx[m];
+        } else {  // MODE_EULER
+            nx[m] = x0 + k[m] * dt;
+            J.x0[o] = nx[m];
+        }
+        J.xs[o] = nx[m];
+        finite = finite && __builtin_isfinite(nx[m]);
+    }
+    if (!finite) atomicOr(J.flag, 1);
+    if (J.prec == PREC_FP64)
+        store_rec<Rec64>((Rec64 *)J.rec, i, nx, J.rec_scale);
+    else
+        store_rec<Rec32>((Rec32 *)J.rec, i, nx, J.rec_scale);
+}
+
+// Items: 256 particles of a one-sided job (one thread each), or 64 particles
+// of a symmetric job.  A symmetric item's particles read up to N / tw
+// source-side slot rows each (10 GB per pass at N = 262,144): its four waves
+// take every fourth row and every fourth target-side chunk, and the four
+// wave sums are added in wave order, so that the item with the most rows is
+// not left to one wave (one thread per particle over all rows took 2.2 ms
+// per pass at N = 262,144, the last block's rows alone).
 __global__ __launch_bounds__(256) void combine_kernel(const JobDev *__restrict__ jobs,
                                                       const CItem *__restrict__ citems,
                                                       const int *__restrict__ ncitems_p,
                                                       int *__restrict__ counters, int mode) {
 #pragma clang fp contract(off)
+    __shared__ double red[3][NCOMP_S][64];  // waves 1-3 of a symmetric item
     const int nci = *ncitems_p;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         counters[0] = 0;
         counters[1] = 0;
     }
     for (int ci = blockIdx.x; ci < nci; ci += gridDim.x) {
-        const CItem c = citems[ci];
+        // the item is the block's: its fields (and the job's) are uniform
+        CItem c = citems[ci];
+        c.job = __builtin_amdgcn_readfirstlane(c.job);
+        c.tile = __builtin_amdgcn_readfirstlane(c.tile);
         const JobDev &J = jobs[c.job];
-        const int i = c.tile * 256 + (int)threadIdx.x;
-        if (i >= J.n) continue;
         const size_t np = (size_t)J.npad;
-        double xs[6];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) xs[m] = J.xs[m * np + i];
-        const double vi = J.vol[i];
-        const double ax = xs[3], ay = xs[4], az = xs[5];
-        double k[6];
-        if (!J.sym) {
-            double S[NCOMP];
-#pragma unroll
-            for (int m = 0; m < NCOMP; ++m) S[m] = 0.0;
-            for (int ch = 0; ch < J.nchunk; ++ch) {
-                const double *p = J.part + (size_t)ch * NCOMP * np + i;
-#pragma unroll
-                for (int m = 0; m < NCOMP; ++m) S[m] += p[m * np];
-            }
-            k[0] = J.su * S[0] + J.uinf[0];
-            k[1] = J.su * S[1] + J.uinf[1];
-            k[2] = J.su * S[2] + J.uinf[2];
-            k[3] = J.sq * S[3] + J.sf * S[6] + J.sd * (vi * S[9] - ax * S[12]);
-            k[4] = J.sq * S[4] + J.sf * S[7] + J.sd * (vi * S[10] - ay * S[12]);
-            k[5] = J.sq * S[5] + J.sf * S[8] + J.sd * (vi * S[11] - az * S[12]);
-        } else {
+        if (J.sym) {
             // target side: the chunks from the one holding this particle's
-            // block start; source side: the blocks up to this particle's own
+            // block start; source side: the blocks up to this particle's
+            // own.  Every particle of the item is in the same block (64
+            // divides tw), so the loops and row addresses are uniform per
+            // wave; lanes past n read padding (inside every row) and are
+            // dropped at the end.
+            const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);  // the wave
+            const int lane = (int)threadIdx.x & 63;
+            const int i0 = c.tile * 64, i = i0 + lane;
+            const bool live = i < J.n;
             const size_t tw = (size_t)J.tw;
-            const int blk = i / J.tw;
+            const int blk = i0 / J.tw;
             const int cfirst = (int)(((size_t)blk * tw) / (size_t)J.chunk_len);
             double S[NCOMP_S], T[NCOMP_S];
 #pragma unroll
             for (int m = 0; m < NCOMP_S; ++m) S[m] = T[m] = 0.0;
-            for (int ch = cfirst; ch < J.nchunk; ++ch) {
-                const double *p = J.part + (size_t)ch * NCOMP_S * np + i;
+            for (int ch = cfirst + w; ch < J.nchunk; ch += 4) {
+                const gdouble *p = (const gdouble *)(J.part + (size_t)ch * NCOMP_S * np + i0) + lane;
 #pragma unroll
                 for (int m = 0; m < NCOMP_S; ++m) S[m] += p[m * np];
             }
-            for (int b = 0; b <= blk; ++b) {
+            // two rows in flight per step (added in row order)
+            int b = w;
+            for (; b + 4 <= blk; b += 8) {
+                const size_t len0 = np - (size_t)b * tw, len1 = len0 - 4 * tw;
+                const gdouble *q0 =
+                    (const gdouble *)(J.jpart + jpart_offset((size_t)b, np, tw) + (i0 - (size_t)b * tw)) +
+                    lane;
+                const gdouble *q1 = (const gdouble *)(J.jpart + jpart_offset((size_t)b + 4, np, tw) +
+                                                      (i0 - ((size_t)b + 4) * tw)) +
+                                    lane;
+                double v0[NCOMP_S], v1[NCOMP_S];
+#pragma unroll
+                for (int m = 0; m < NCOMP_S; ++m) {
+                    v0[m] = q0[m * len0];
+                    v1[m] = q1[m * len1];
+                }
+#pragma unroll
+                for (int m = 0; m < NCOMP_S; ++m) T[m] = (T[m] + v0[m]) + v1[m];
+            }
+            if (b <= blk) {
                 const size_t len = np - (size_t)b * tw;
-                const double *q = J.jpart + jpart_offset((size_t)b, np, tw) + (i - (size_t)b * tw);
+                const gdouble *q =
+                    (const gdouble *)(J.jpart + jpart_offset((size_t)b, np, tw) + (i0 - (size_t)b * tw)) +
+                    lane;
 #pragma unroll
                 for (int m = 0; m < NCOMP_S; ++m) T[m] += q[m * len];
             }
+            // source-side stretching enters with a minus sign
             double U[NCOMP_S];
 #pragma unroll
-            for (int m = 0; m < NCOMP_S; ++m) U[m] = S[m] + T[m];
-            U[3] = S[3] - T[3];
-            U[4] = S[4] - T[4];
-            U[5] = S[5] - T[5];
-            k[0] = J.su * U[0] + J.uinf[0];
-            k[1] = J.su * U[1] + J.uinf[1];
-            k[2] = J.su * U[2] + J.uinf[2];
-            k[3] = J.sq * U[3] + J.sd * (vi * U[6] - ax * U[9]);
-            k[4] = J.sq * U[4] + J.sd * (vi * U[7] - ay * U[9]);
-            k[5] = J.sq * U[5] + J.sd * (vi * U[8] - az * U[9]);
-        }
-        bool finite = true;
+            for (int m = 0; m < NCOMP_S; ++m) U[m] = (m >= 3 && m < 6) ? S[m] - T[m] : S[m] + T[m];
+            if (w > 0) {
 #pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            J.rate[m * np + i] = k[m];
-            finite = finite && __builtin_isfinite(k[m]);
-        }
-        if (!finite) atomicOr(J.flag, 1);
-        if (mode == MODE_SOLVE) continue;
-        const double dt = J.dt;
-        double nx[6];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            const size_t o = m * np + i;
-            const double x0 = J.x0[o];
-            if (mode == MODE_RK4_1) {
-                J.k1[o] = k[m];
-                nx[m] = k[m] * (0.5 * dt) + x0;
-            } else if (mode == MODE_RK4_2) {
-                J.k23[o] = k[m];
-                nx[m] = k[m] * (0.5 * dt) + x0;
-            } else if (mode == MODE_RK4_3) {
-                J.k23[o] = J.k23[o] + k[m];
-                nx[m] = k[m] * dt + x0;
-            } else if (mode == MODE_RK4_4) {
-                const double a = (J.k1[o] + k[m]) * (dt / 6.);
-                const double b = J.k23[o] * (dt / 3.);
-                nx[m] = x0 + (a + b);
-                J.x0[o] = nx[m];
-            } else {  // MODE_EULER
-                nx[m] = x0 + k[m] * dt;
-                J.x0[o] = nx[m];
+                for (int m = 0; m < NCOMP_S; ++m) red[w - 1][m][lane] = U[m];
             }
-            J.xs[o] = nx[m];
-            finite = finite && __builtin_isfinite(nx[m]);
+            __syncthreads();
+            if (w == 0 && live) {
+#pragma unroll
+                for (int m = 0; m < NCOMP_S; ++m)
+                    U[m] = ((U[m] + red[0][m][lane]) + red[1][m][lane]) + red[2][m][lane];
+                const double vi = J.vol[i];
+                const double ax = J.xs[3 * np + i], ay = J.xs[4 * np + i], az = J.xs[5 * np + i];
+                double k[6];
+                k[0] = J.su * U[0] + J.uinf[0];
+                k[1] = J.su * U[1] + J.uinf[1];
+                k[2] = J.su * U[2] + J.uinf[2];
+                k[3] = J.sq * U[3] + J.sd * (vi * U[6] - ax * U[9]);
+                k[4] = J.sq * U[4] + J.sd * (vi * U[7] - ay * U[9]);
+                k[5] = J.sq * U[5] + J.sd * (vi * U[8] - az * U[9]);
+                finish_particle(J, i, k, mode);
+            }
+            __syncthreads();  // red is rewritten by the next symmetric item
+            continue;
         }
-        if (!finite) atomicOr(J.flag, 1);
-        if (J.prec == PREC_FP64)
-            store_rec<Rec64>((Rec64 *)J.rec, i, nx, J.rec_scale);
-        else
-            store_rec<Rec32>((Rec32 *)J.rec, i, nx, J.rec_scale);
+        const int i = c.tile * 256 + (int)threadIdx.x;
+        if (i >= J.n) continue;
+        const double vi = J.vol[i];
+        const double ax = J.xs[3 * np + i], ay = J.xs[4 * np + i], az = J.xs[5 * np + i];
+        double S[NCOMP];
+#pragma unroll
+        for (int m = 0; m < NCOMP; ++m) S[m] = 0.0;
+        for (int ch = 0; ch < J.nchunk; ++ch) {
+            const double *p = J.part + (size_t)ch * NCOMP * np + i;
+#pragma unroll
+            for (int m = 0; m < NCOMP; ++m) S[m] += p[m * np];
+        }
+        double k[6];
+        k[0] = J.su * S[0] + J.uinf[0];
+        k[1] = J.su * S[1] + J.uinf[1];
+        k[2] = J.su * S[2] + J.uinf[2];
+        k[3] = J.sq * S[3] + J.sf * S[6] + J.sd * (vi * S[9] - ax * S[12]);
+        k[4] = J.sq * S[4] + J.sf * S[7] + J.sd * (vi * S[10] - ay * S[12]);
+        k[5] = J.sq * S[5] + J.sf * S[8] + J.sd * (vi * S[11] - az * S[12]);
+        finish_particle(J, i, k, mode);
     }
 }
 

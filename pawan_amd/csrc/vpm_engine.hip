@@ -511,12 +511,40 @@ void Engine::build_plan() {
                 }
             }
         }
-        for (int t = 0; t < (int)((j.n + 255) / 256); ++t) ci.push_back(CItem{(int)k, t});
+        if (j.sym) {  // 64 particles per item
+            for (int t = 0; t < (int)((j.n + 63) / 64); ++t) ci.push_back(CItem{(int)k, t});
+        } else {
+            for (int t = 0; t < (int)((j.n + 255) / 256); ++t) ci.push_back(CItem{(int)k, t});
+        }
     }
     // larger items first (their cost is tile x chunk length): shorter tail
     auto bigger = [](const Item &a, const Item &b) { return a.pad > b.pad; };
     std::stable_sort(it64.begin(), it64.end(), bigger);
     std::stable_sort(it32.begin(), it32.end(), bigger);
+    // Combine items: the combine grid takes item k, k + G, k + 2G, ... per
+    // block (G = gridc_ blocks, all resident); sorted by the values each
+    // reads, largest first, and every second round of G reversed, so that
+    // each block's items add up to about the same work.
+    {
+        auto cost = [&](const CItem &c) -> size_t {
+            const Job &j = *jobs_[(size_t)c.job];
+            if (!j.sym) return (size_t)NCOMP * 256 * (size_t)j.nchunk;
+            const size_t b0 = (size_t)c.tile * 64 / (size_t)j.tw;
+            const size_t cf = b0 * (size_t)j.tw / (size_t)j.chunk_len;
+            return (size_t)NCOMP_S * 64 * (b0 + 1 + (size_t)j.nchunk - cf);
+        };
+        std::vector<std::pair<size_t, CItem>> kc;
+        kc.reserve(ci.size());
+        for (const CItem &c : ci) kc.push_back({cost(c), c});
+        std::stable_sort(kc.begin(), kc.end(),
+                         [](const std::pair<size_t, CItem> &a, const std::pair<size_t, CItem> &b) {
+                             return a.first > b.first;
+                         });
+        const size_t G = (size_t)std::max(1, gridc_);
+        for (size_t r = 0; r * G < kc.size(); ++r)
+            if (r & 1) std::reverse(kc.begin() + r * G, kc.begin() + std::min(kc.size(), (r + 1) * G));
+        for (size_t k = 0; k < ci.size(); ++k) ci[k] = kc[k].second;
+    }
     n64_ = (int)it64.size();
     n32_ = (int)it32.size();
     nci_ = (int)ci.size();

@@ -110,6 +110,8 @@ __host__ __device__ inline size_t jpart_offset(size_t b, size_t npad, size_t tw)
 struct Item {
     int job, tile, chunk, pad;
 };
+// Combine item: particles [256 tile, 256 tile + 256) of a one-sided job,
+// [64 tile, 64 tile + 64) of a symmetric one.
 struct CItem {
     int job, tile;
 };
