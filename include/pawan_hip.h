@@ -71,7 +71,9 @@ typedef struct pw_config {
     int precision;    /* PW_FP64 / PW_FP32C                                      */
     double nu;        /* kinematic viscosity (reference: 2.5e-3, interaction.cpp:13) */
     double uinf[3];   /* uniform freestream added to every velocity (0 = none)     */
-    int block;        /* threads per workgroup 64/128/256; 0 = tuned or default  */
+    int block;        /* threads per workgroup 64/128/256; 0 = tuned or default;
+                         512 for the binary32 symmetric HIGHORDER pass at one
+                         target per thread (half the source-side slot rows)    */
     int chunk;        /* sources per work item (multiple of block); 0 = policy   */
     int targets_per_thread; /* 1 or 2; 0 = tuned or default                      */
     int unroll;       /* source-loop unroll 1/2/4; 0 = tuned or default          */
@@ -80,7 +82,9 @@ typedef struct pw_config {
 
 /* A compiled kernel variant of the pair pass. */
 typedef struct pw_variant {
-    int block;              /* 64, 128 or 256 threads per workgroup               */
+    int block;              /* 64, 128 or 256 threads per workgroup; 512: binary32
+                               symmetric HIGHORDER jobs at one target per thread
+                               (binary64 jobs of that engine run at 256)       */
     int targets_per_thread; /* 1 or 2                                             */
     int unroll;             /* 1, 2 or 4; applies to the precision being tuned    */
     int scheme;             /* PW_SCHEME_* (0: default)                            */

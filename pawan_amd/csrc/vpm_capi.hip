@@ -89,6 +89,7 @@ struct pw_ctx {
         pw::Variant v;
         v.block = cfg.block ? cfg.block : default_block(total());
         v.tpt = cfg.targets_per_thread ? cfg.targets_per_thread : default_tpt(total(), cfg.precision, cfg.kernel);
+        if (v.block == 512) v.tpt = 1;  // the one 512-thread variant
         if (cfg.unroll) v.unroll64 = v.unroll32 = cfg.unroll;
         v.sym32 = cfg.scheme != PW_SCHEME_ONESIDED;
         v.sym64 = cfg.scheme == PW_SCHEME_SYMMETRIC;
@@ -177,8 +178,12 @@ int pw_create(pw_ctx **out, const pw_config *cfg) {
             throw Error(PW_EINVAL, "unknown kernel");
         if (c.precision != PW_FP64 && c.precision != PW_FP32C)
             throw Error(PW_EINVAL, "unknown precision");
-        if (c.block != 0 && c.block != 64 && c.block != 128 && c.block != 256)
-            throw Error(PW_EINVAL, "block must be 0, 64, 128 or 256");
+        if (c.block != 0 && c.block != 64 && c.block != 128 && c.block != 256 && c.block != 512)
+            throw Error(PW_EINVAL, "block must be 0, 64, 128, 256 or 512");
+        if (c.block == 512 && (c.precision != PW_FP32C || c.kernel != PW_KERNEL_HIGHORDER ||
+                               c.scheme == PW_SCHEME_ONESIDED || c.targets_per_thread == 2))
+            throw Error(PW_EINVAL,
+                        "block 512 is the binary32 symmetric HIGHORDER pass, one target per thread");
         if (c.chunk < 0 || (c.chunk && c.block && c.chunk % c.block))
             throw Error(PW_EINVAL, "chunk must be a non-negative multiple of block");
         if (c.targets_per_thread < 0 || c.targets_per_thread > 2)
@@ -815,7 +820,8 @@ int pw_autotune(int device, int kernel, int precision, size_t n, const double *p
     if (!best || !pos || !vort || !radius || !volume || n == 0) return PW_EINVAL;
     return guarded(nullptr, [&] {
         if (precision != PW_FP64 && precision != PW_FP32C) throw Error(PW_EINVAL, "precision");
-        const int blocks[3] = {64, 128, 256}, tpts[2] = {1, 2}, unrs[3] = {1, 2, 4};
+        // 512: binary32 symmetric pass only (binary64 jobs would run at 256)
+        const int blocks[4] = {64, 128, 256, 512}, tpts[2] = {1, 2}, unrs[3] = {1, 2, 4};
         // binary32: the symmetric pass (no unroll parameter) and the one-sided one
         std::vector<int> schemes;
         schemes = {PW_SCHEME_SYMMETRIC, PW_SCHEME_ONESIDED};
@@ -826,6 +832,7 @@ int pw_autotune(int device, int kernel, int precision, size_t n, const double *p
                 for (int u : unrs) {
                     const bool sym = sc == PW_SCHEME_SYMMETRIC;
                     if (sym && u != unrs[0]) continue;
+                    if (b == 512 && (precision != PW_FP32C || !sym)) continue;
                     pw::Variant v;
                     v.block = b;
                     v.tpt = t;

@@ -115,7 +115,12 @@ size_t jpart_count(size_t n, size_t npad, size_t tw) {
     return jpart_offset(nblk, npad, tw);
 }
 
+int block_for(int block, int precision) {
+    return precision == PREC_FP64 && block > 256 ? 256 : block;
+}
+
 size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym) {
+    TB = block_for(TB, precision);
     size_t npad = round_up(std::max<size_t>(n, 1), 256);
     size_t L = sym ? chunk_for_sym(n, TB, TPT) : chunk_for(n, TB);
     size_t nchunk = (n + L - 1) / L;
@@ -211,6 +216,7 @@ static PairFn pair_fn_sym(int kind, int tb, int tpt) {
         PW_CASES(KIND_HIGHORDER, 64, 1) PW_CASES(KIND_HIGHORDER, 64, 2)
         PW_CASES(KIND_HIGHORDER, 128, 1) PW_CASES(KIND_HIGHORDER, 128, 2)
         PW_CASES(KIND_HIGHORDER, 256, 1) PW_CASES(KIND_HIGHORDER, 256, 2)
+        PW_CASES(KIND_HIGHORDER, 512, 1)
     } else {
         PW_CASES(KIND_GAUSSIAN, 64, 1) PW_CASES(KIND_GAUSSIAN, 128, 1)
         PW_CASES(KIND_GAUSSIAN, 256, 1)
@@ -248,21 +254,22 @@ static PairFn pair_fn(int prec, int kind, int tb, int tpt, int unr) {
 
 bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32, int sym32, int sym64,
                     int tpt32) {
-    const int t32 = tpt32 ? tpt32 : tpt;
-    return (sym64 ? pair_fn_sym64(kind, tb, tpt) != nullptr
-                  : pair_fn(PREC_FP64, kind, tb, tpt, unr64) != nullptr) &&
+    const int t32 = tpt32 ? tpt32 : tpt, tb64 = block_for(tb, PREC_FP64);
+    return (sym64 ? pair_fn_sym64(kind, tb64, tpt) != nullptr
+                  : pair_fn(PREC_FP64, kind, tb64, tpt, unr64) != nullptr) &&
            (sym32 ? pair_fn_sym(kind, tb, t32) != nullptr
                   : pair_fn(PREC_FP32C, kind, tb, t32, unr32) != nullptr);
 }
 
 Engine::Engine(int device, int kind, const Variant &v)
-    : device_(device), kind_(kind), tb_(v.block), tpt_(v.tpt), unr64_(v.unroll64),
-      unr32_(v.unroll32), sym32_(v.sym32 ? 1 : 0), sym64_(v.sym64 ? 1 : 0),
-      tpt32_(v.tpt32 ? v.tpt32 : v.tpt) {
-    if (tb_ != 64 && tb_ != 128 && tb_ != 256) throw Error(PW_EINVAL, "block must be 64, 128 or 256");
+    : device_(device), kind_(kind), tb_(block_for(v.block, PREC_FP64)), tpt_(v.tpt),
+      unr64_(v.unroll64), unr32_(v.unroll32), sym32_(v.sym32 ? 1 : 0), sym64_(v.sym64 ? 1 : 0),
+      tb32_(block_for(v.block, PREC_FP32C)), tpt32_(v.tpt32 ? v.tpt32 : v.tpt) {
+    if (tb32_ != 64 && tb32_ != 128 && tb32_ != 256 && tb32_ != 512)
+        throw Error(PW_EINVAL, "block must be 64, 128, 256 or 512");
     if (kind_ != KIND_HIGHORDER && kind_ != KIND_GAUSSIAN) throw Error(PW_EINVAL, "unknown kernel");
     fn64_ = sym64_ ? pair_fn_sym64(kind_, tb_, tpt_) : pair_fn(PREC_FP64, kind_, tb_, tpt_, unr64_);
-    fn32_ = sym32_ ? pair_fn_sym(kind_, tb_, tpt32_) : pair_fn(PREC_FP32C, kind_, tb_, tpt32_, unr32_);
+    fn32_ = sym32_ ? pair_fn_sym(kind_, tb32_, tpt32_) : pair_fn(PREC_FP32C, kind_, tb32_, tpt32_, unr32_);
     if (!fn64_ || !fn32_) throw Error(PW_EINVAL, "no compiled kernel variant for these parameters");
     int ndev = 0;
     PW_HIP(hipGetDeviceCount(&ndev));
@@ -272,7 +279,7 @@ Engine::Engine(int device, int kind, const Variant &v)
     PW_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device_));
     int b64 = 0, b32 = 0;
     PW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b64, (const void *)fn64_, tb_, 0));
-    PW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b32, (const void *)fn32_, tb_, 0));
+    PW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b32, (const void *)fn32_, tb32_, 0));
     grid64_ = std::max(1, b64) * cus_;
     grid32_ = std::max(1, b32) * cus_;
     gridc_ = 4 * cus_;
@@ -318,13 +325,13 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     j->par = par;
     j->kind = kind_;
     j->sym = par.precision == PREC_FP32C ? sym32_ != 0 : sym64_ != 0;
-    const int tpt = tpt_of(par.precision);
+    const int tpt = tpt_of(par.precision), tb = tb_of(par.precision);
     size_t L = chunk_override ? chunk_override
-                              : (j->sym ? chunk_for_sym(n, tb_, tpt) : chunk_for(n, tb_));
-    if (L % (size_t)tb_ != 0) throw Error(PW_EINVAL, "chunk must be a multiple of the block size");
+                              : (j->sym ? chunk_for_sym(n, tb, tpt) : chunk_for(n, tb));
+    if (L % (size_t)tb != 0) throw Error(PW_EINVAL, "chunk must be a multiple of the block size");
     j->chunk_len = (int)L;
     j->nchunk = (int)((n + L - 1) / L);
-    j->ntile = (int)((n + tb_ * tpt - 1) / (tb_ * tpt));
+    j->ntile = (int)((n + tb * tpt - 1) / (tb * tpt));
     j->chunk_override = chunk_override;
     j->part_chunks = j->nchunk;
     j->uniform = true;
@@ -340,7 +347,7 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     j->part.reset((size_t)j->nchunk * NCOMP * np * sizeof(double));
     j->vol.reset(2 * np * sizeof(double));  // volume | radius
     j->flag.reset(256);
-    j->tw = tb_ * tpt;
+    j->tw = tb * tpt;
     alloc_jpart(j.get());
     j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + j->flag.bytes +
                j->jpart.bytes;
@@ -441,12 +448,12 @@ void Engine::append(Job *j, size_t m, const double *pos, const double *vort, con
                           hipMemcpyHostToDevice, stream_));
     PW_HIP(hipStreamSynchronize(stream_));
     j->n = n2;
-    const int tpt = tpt_of(j->par.precision);
+    const int tpt = tpt_of(j->par.precision), tb = tb_of(j->par.precision);
     const size_t L = j->chunk_override ? j->chunk_override
-                                       : (j->sym ? chunk_for_sym(n2, tb_, tpt) : chunk_for(n2, tb_));
+                                       : (j->sym ? chunk_for_sym(n2, tb, tpt) : chunk_for(n2, tb));
     j->chunk_len = (int)L;
     j->nchunk = (int)((n2 + L - 1) / L);
-    j->ntile = (int)((n2 + tb_ * tpt - 1) / (tb_ * tpt));
+    j->ntile = (int)((n2 + tb * tpt - 1) / (tb * tpt));
     if (j->nchunk > j->part_chunks) {
         j->part_chunks = std::max(j->nchunk, 16);
         j->part.reset((size_t)j->part_chunks * NCOMP * np * sizeof(double));
@@ -609,7 +616,8 @@ void Engine::launch_stage(int mode, bool timed) {
             b = ev_pool_[k + 1];
             PW_HIP(hipEventRecord(a, stream_));
         }
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(tb_), 0, stream_, d_jobs_, items, nitems, queue);
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(tb_of(prec)), 0, stream_, d_jobs_, items, nitems,
+                           queue);
         PW_HIP(hipGetLastError());
         if (timed) {
             PW_HIP(hipEventRecord(b, stream_));

@@ -70,6 +70,11 @@ size_t chunk_for_sym(size_t n, int TB, int TPT);
 // Bytes a resident job occupies on the device (TB x TPT targets per block;
 // sym: the job uses the symmetric pass and its source-side slots).
 size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym);
+// Threads per workgroup of a precision's jobs in an engine built with `block`:
+// 512 exists only for the binary32 symmetric HIGHORDER pass at one target per
+// thread (half the source-side slot rows of 256), so an engine asked for 512
+// runs its binary64 jobs at 256.
+int block_for(int block, int precision);
 // Binary64 sums in a symmetric job's source-side slots (block rows of
 // NCOMP_S x (npad - b tw) for every block b).
 size_t jpart_count(size_t n, size_t npad, size_t tw);
@@ -121,7 +126,7 @@ class Engine {
     int device() const { return device_; }
     int kind() const { return kind_; }
     int block() const { return tb_; }
-    Variant variant() const { return Variant{tb_, tpt_, unr64_, unr32_, sym32_, sym64_, tpt32_}; }
+    Variant variant() const { return Variant{tb32_, tpt_, unr64_, unr32_, sym32_, sym64_, tpt32_}; }
     hipStream_t stream() const { return stream_; }
 
     // Create a job from N x 3 positions/vorticities and N radius/volume.
@@ -177,6 +182,8 @@ class Engine {
     double pairs_per_rhs() const;
 
     int device_, kind_, tb_, tpt_, unr64_, unr32_, sym32_, sym64_;
+    int tb32_;  // threads per workgroup of binary32 jobs (tb_: binary64 jobs)
+    int tb_of(int precision) const { return precision == PREC_FP32C ? tb32_ : tb_; }
     int tpt32_;  // targets per thread of binary32 jobs (tpt_: binary64 jobs)
     int tpt_of(int precision) const { return precision == PREC_FP32C ? tpt32_ : tpt_; }
     void alloc_jpart(Job *j);

@@ -85,6 +85,16 @@ def test_invalid_config_rejected_before_device_use():
         setattr(c2, field, bad)
         assert lib.pw_create(C.byref(h), C.byref(c2)) == _lib.PW_EINVAL
         assert h.value is None
+    # block 512 exists only for the binary32 symmetric HIGHORDER pass at one
+    # target per thread
+    for changes in ({"precision": _lib.FP64}, {"scheme": _lib.SCHEME_ONESIDED},
+                    {"kernel": _lib.KERNEL_GAUSSIAN}, {"targets_per_thread": 2}):
+        c2 = _lib.PwConfig.from_buffer_copy(cfg)
+        c2.block, c2.precision = 512, _lib.FP32C
+        for field, v in changes.items():
+            setattr(c2, field, v)
+        assert lib.pw_create(C.byref(h), C.byref(c2)) == _lib.PW_EINVAL, changes
+        assert h.value is None
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")

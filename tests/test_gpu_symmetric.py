@@ -41,7 +41,7 @@ def _rates(w, nu, precision, scheme=pa.SCHEME_DEFAULT, wake_n=None, **kw):
 
 
 @pytest.mark.parametrize("n", [2, 3, 63, 64, 65, 127, 200, 511, 513, 1000])
-@pytest.mark.parametrize("block,tpt", [(64, 1), (128, 2), (256, 1), (256, 2)])
+@pytest.mark.parametrize("block,tpt", [(64, 1), (128, 2), (256, 1), (256, 2), (512, 1)])
 def test_symmetric_tile_edges(n, block, tpt):
     """Random blobs: every pair counted once.  Random strengths make the sums
     cancel far more than any vortex workload does, so the binary32 rounding
@@ -51,7 +51,8 @@ def test_symmetric_tile_edges(n, block, tpt):
     print(f"case symmetric n={n} block={block} tpt={tpt}", flush=True)
     nu = 1e-3
     vel, dv = _rates(w, nu, pa.FP32C, block=block, targets_per_thread=tpt)
-    vo, do = _rates(w, nu, pa.FP32C, pa.SCHEME_ONESIDED, block=block, targets_per_thread=tpt)
+    ob = min(block, 256)  # the one-sided pass has no 512-thread variant
+    vo, do = _rates(w, nu, pa.FP32C, pa.SCHEME_ONESIDED, block=ob, targets_per_thread=tpt)
     rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL)
     ev, ed = rel_rms(vel, rv), rel_rms(dv, rd)
     print(f"  symmetric vel {ev:.2e} dvort {ed:.2e}; one-sided vel {rel_rms(vo, rv):.2e} "
@@ -137,6 +138,32 @@ def test_symmetric64_wakes_nonuniform_gaussian_straddle():
         ev, ed = per_particle_rel(vel, rv), per_particle_rel(dv, rd)
         print(f"  vel {ev:.2e} dvort {ed:.2e}", flush=True)
         assert ev < 1e-10 and ed < 1e-10
+
+
+def test_block512_batch_alone_equals_mixed():
+    """A batch built with 512 threads runs its binary32 jobs on the 512 x 1
+    symmetric pass (half the source-side slot rows) and its binary64 jobs at
+    256: every job bitwise the same alone and in the mix, the binary32 job
+    within 1e-6 of the oracle after the steps."""
+    jobs = gen.batch_jobs(3, seed=11, n_min=600, n_max=2500)
+    precs = [pa.FP32C, pa.FP64, pa.FP32C]
+    print("case batch block 512: n = " + ", ".join(str(j["n"]) for j in jobs), flush=True)
+    res = []
+    for members in ([0], [0, 1, 2]):
+        B = pa.Batch(block=512, targets_per_thread=1)
+        ids = [B.add(jobs[k], jobs[k]["nu"], jobs[k]["dt"], precs[k]) for k in members]
+        B.step(2, graph=True)
+        res.append(B.get(ids[0]))
+        B.close()
+    assert np.array_equal(res[0]["pos"], res[1]["pos"])
+    assert np.array_equal(res[0]["vort"], res[1]["vort"])
+    j = jobs[0]
+    x = orc.rk4(orc.pack_state(j["pos"], j["vort"]), j["radius"], j["volume"], j["nu"], j["dt"], 2,
+                mode=orc.OMP_CLEAN)
+    p, v = orc.unpack_state(x)
+    ep, ev = rel_rms(res[0]["pos"], p), rel_rms(res[0]["vort"], v)
+    print(f"  binary32 512 x 1 after 2 steps vs oracle: pos {ep:.2e} vort {ev:.2e}", flush=True)
+    assert ep < TOL32 and ev < TOL32
 
 
 def test_symmetric64_rk4_batch_alone_equals_mixed():

@@ -70,4 +70,7 @@ def test_autotune_fp32c_both_schemes(tmp_path):
     print(f"  best {best}", flush=True)
     schemes = {e["scheme"] for e in allv}
     assert schemes == {pa.SCHEME_SYMMETRIC, pa.SCHEME_ONESIDED}
-    assert len(allv) == 6 + 18 and best["pair_ms"] == min(e["pair_ms"] for e in allv)
+    # symmetric: 3 blocks x 2 targets per thread, plus 512 x 1; one-sided: 3 x 2 x 3 unrolls
+    assert len(allv) == 7 + 18 and best["pair_ms"] == min(e["pair_ms"] for e in allv)
+    big = [(e["targets_per_thread"], e["scheme"]) for e in allv if e["block"] == 512]
+    assert big == [(1, pa.SCHEME_SYMMETRIC)]
