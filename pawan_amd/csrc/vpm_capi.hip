@@ -45,17 +45,25 @@ int guarded(std::string *err, F &&f) {
     }
 }
 
-int default_block(size_t n) {
+// One target per thread is the faster binary32 symmetric pass (59.9 against
+// 68.2 ms at N = 262,144; 244 against 270 ms at 524,288), but its source-side
+// slots take 40 N^2 / (block x targets per thread) bytes: past N = 524,288
+// (43 GB at 256 threads) a binary32 symmetric HIGHORDER job takes 512 threads x
+// 1 target, which halves them as 256 x 2 does with the faster one-target pass
+// (973 against 1,052 ms per RHS at N = 1,048,576).  From 131,072 to 524,288
+// 512 x 1 and 256 x 1 are within 1% of each other (the autotuner picks).
+int default_block(size_t n, int precision, int kernel, int scheme) {
+    if (precision == PW_FP32C && kernel == PW_KERNEL_HIGHORDER && scheme != PW_SCHEME_ONESIDED &&
+        n > 524288)
+        return 512;
     if (n >= 32768) return 256;
     if (n >= 8192) return 128;
     return 64;
 }
 
-// One target per thread is the faster binary32 symmetric pass (59.9 against
-// 68.2 ms at N = 262,144; 244 against 270 ms at 524,288), but its source-side
-// slots take 40 N^2 / (block x targets per thread) bytes: past N = 524,288
-// (43 GB at 256 threads) two targets per thread halve them.  (The GAUSSIAN
-// kernel is compiled with one target per thread only.)
+// Two targets per thread for the binary32 one-sided HIGHORDER pass past
+// N = 524,288 (the symmetric one takes 512 x 1 there).  (The GAUSSIAN kernel is
+// compiled with one target per thread only.)
 int default_tpt(size_t n, int precision, int kernel) {
     return kernel == PW_KERNEL_HIGHORDER && precision == PW_FP32C && n > 524288 ? 2 : 1;
 }
@@ -87,7 +95,8 @@ struct pw_ctx {
 
     pw::Variant choose_variant() const {
         pw::Variant v;
-        v.block = cfg.block ? cfg.block : default_block(total());
+        v.block = cfg.block ? cfg.block
+                            : default_block(total(), cfg.precision, cfg.kernel, cfg.scheme);
         v.tpt = cfg.targets_per_thread ? cfg.targets_per_thread : default_tpt(total(), cfg.precision, cfg.kernel);
         if (v.block == 512) v.tpt = 1;  // the one 512-thread variant
         if (cfg.unroll) v.unroll64 = v.unroll32 = cfg.unroll;
@@ -637,8 +646,10 @@ struct pw_server {
 extern "C" {
 
 size_t pw_job_footprint(size_t n, int precision) {
-    return pw::job_footprint(n, precision, default_block(n),
-                             default_tpt(n, precision, PW_KERNEL_HIGHORDER), precision == PW_FP32C);
+    const int b = default_block(n, precision, PW_KERNEL_HIGHORDER, PW_SCHEME_DEFAULT);
+    return pw::job_footprint(n, precision, b,
+                             b == 512 ? 1 : default_tpt(n, precision, PW_KERNEL_HIGHORDER),
+                             precision == PW_FP32C);
 }
 
 int pw_server_create(pw_server **out, int device, int kernel, size_t hbm_budget_bytes,

@@ -50,9 +50,10 @@ def test_host_only_entry_points():
     assert lib.pw_job_footprint(262144, 0) > 262144 * 64
     # binary32 records carry binary64 strengths too: 80 B instead of 64 B
     # binary32: 16 more bytes per record, plus the symmetric pass's source-side
-    # slots, NCOMP_S = 10 binary64 sums per (particle, block of 256 x TPT targets)
-    # for the blocks at or before the particle's own; TPT = 1 up to 524,288
-    # particles, 2 beyond.  The symmetric pass also takes 16-64 source chunks
+    # slots, NCOMP_S = 10 binary64 sums per (particle, block of targets) for
+    # the blocks at or before the particle's own; blocks of 256 targets
+    # (256 x 1) up to 524,288 particles, 512 (512 x 1) beyond.  The
+    # symmetric pass also takes 16-64 source chunks
     # (enough that its triangle of work items is ~8,192 long) where the
     # one-sided pass takes 16: 13 binary64 partial sums per chunk and particle.
     for n, tw, nch in ((262144, 256, 16), (524288, 256, 16), (786432, 512, 16), (65536, 256, 64),

@@ -57,13 +57,27 @@ def test_fullsize_fp32c_within_1e6(wake):
 
 
 def test_twice_fullsize_fp32c_default_within_1e6():
-    """N = 524,288, the largest size whose binary32 default keeps one target
-    per thread (43 GB of source-side slots): the symmetric pass stays within
+    """N = 524,288, the largest size whose binary32 default keeps 256-target
+    blocks (43 GB of source-side slots): the symmetric pass stays within
     1e-6 of the binary64 path and conserves the total vorticity rate to
     binary32 rounding (a pair's +s and -s are summed in different binary32
     runs: measured 6e-8 of the summed magnitudes)."""
     w = gen.config_rotor_wake(2048)
     print(f"case N={w['n']} binary32 default variant", flush=True)
+    v, d = _rates(w)
+    v32, d32 = _rates(w, precision=pa.FP32C)
+    ev, ed = rel_rms(v32, v), rel_rms(d32, d)
+    print(f"  fp32c vs fp64: velocity {ev:.3e} vorticity rate {ed:.3e}", flush=True)
+    assert ev < 1e-6 and ed < 1e-6
+    assert np.abs(d32.sum(axis=0)).max() <= 1e-6 * np.abs(d32).sum()
+
+
+def test_fullsize_fp32c_default_beyond_524288():
+    """N = 786,432: past 524,288 particles the binary32 default is the 512 x 1
+    symmetric pass (one slot row per 512 targets, 48 GB here): within 1e-6 of
+    the binary64 path, total vorticity rate conserved to binary32 rounding."""
+    w = gen.config_rotor_wake(3072)
+    print(f"case N={w['n']} binary32 default variant (512 x 1)", flush=True)
     v, d = _rates(w)
     v32, d32 = _rates(w, precision=pa.FP32C)
     ev, ed = rel_rms(v32, v), rel_rms(d32, d)
