@@ -73,7 +73,8 @@ typedef struct pw_config {
     double uinf[3];   /* uniform freestream added to every velocity (0 = none)     */
     int block;        /* threads per workgroup 64/128/256; 0 = tuned or default;
                          512 for the binary32 symmetric HIGHORDER pass at one
-                         target per thread (half the source-side slot rows)    */
+                         target per thread (half the source-side slot rows; the
+                         default for such a context past 524,288 particles)    */
     int chunk;        /* sources per work item (multiple of block); 0 = policy   */
     int targets_per_thread; /* 1 or 2; 0 = tuned or default                      */
     int unroll;       /* source-loop unroll 1/2/4; 0 = tuned or default          */
