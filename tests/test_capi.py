@@ -124,3 +124,9 @@ def test_tune_cache_roundtrip(tmp_path):
     assert (v.block, v.targets_per_thread, v.unroll, v.scheme) == (64, 2, 0, _lib.SCHEME_SYMMETRIC)
     assert lib.pw_tune_lookup(path, 1, 0, 262144, C.byref(v)) == _lib.PW_ENOTFOUND
     assert len(open(path.decode()).read().splitlines()) == 2
+    # the 512-thread binary32 symmetric variant the tuner picks at N = 262,144
+    e4 = _lib.PwTuneEntry(_lib.PwVariant(512, 1, 0, _lib.SCHEME_SYMMETRIC), 60.6, 1.13e12)
+    assert lib.pw_tune_save(path, 0, 1, 262144, C.byref(e4)) == 0
+    assert lib.pw_tune_lookup(path, 0, 1, 262144, C.byref(v)) == 0
+    assert (v.block, v.targets_per_thread, v.unroll, v.scheme) == (512, 1, 0, _lib.SCHEME_SYMMETRIC)
+    assert len(open(path.decode()).read().splitlines()) == 3
