@@ -37,6 +37,8 @@ def load():
                                 _dp, _dp, _dp]),
         "orc_rates_split_ld": (C.c_int, [C.c_int, C.c_double, C.c_size_t, _dp, _dp, _dp, _dp,
                                          _dp, _dp, _dp]),
+        "orc_rates_sample": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_size_t, _dp, _dp, _dp,
+                                       _dp, _dp, C.c_size_t, _sp, _dp, _dp, _dp]),
         "orc_pack_state": (None, [C.c_size_t, _sp, _dp, _dp, _dp]),
         "orc_unpack_state": (None, [C.c_size_t, _sp, _dp, _dp, _dp]),
         "orc_rk4": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_size_t, _sp, _dp, _dp, _dp, _dp,
@@ -108,6 +110,27 @@ def rates_split(pos, vort, radius, volume, nu, kind=HIGHORDER):
     load().orc_rates_split_ld(kind, nu, n, _p(pos), _p(vort), _p(radius), _p(volume), _p(vel),
                               _p(st), _p(df))
     return vel, st, df
+
+
+def rates_sample(pos, vort, radius, volume, nu, targets, kind=HIGHORDER, ld=False, uinf=None,
+                 split=True):
+    """Rates of the particles `targets` against all particles (wakes
+    concatenated): (vel, stretching, diffusion), or (vel, dvort) with
+    split=False, each len(targets) x 3.  ld=False: the reference's binary64
+    pair arithmetic summed in source order; ld=True: long double."""
+    pos, vort, radius, volume = _f(pos), _f(vort), _f(radius), _f(volume)
+    n = radius.size
+    t = np.ascontiguousarray(np.asarray(targets, dtype=np.uintp))
+    ns = t.size
+    vel, st = np.zeros((ns, 3)), np.zeros((ns, 3))
+    df = np.zeros((ns, 3)) if split else None
+    u = None if uinf is None else _f(uinf)
+    rc = load().orc_rates_sample(1 if ld else 0, kind, nu, n, _p(pos), _p(vort), _p(radius),
+                                 _p(volume), _p(u), ns, t.ctypes.data_as(_sp), _p(vel), _p(st),
+                                 _p(df))
+    if rc:
+        raise ValueError("orc_rates_sample failed")
+    return (vel, st, df) if split else (vel, st)
 
 
 def pack_state(pos, vort, wake_n=None):

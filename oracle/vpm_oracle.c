@@ -133,6 +133,10 @@ void orc_kernel_ld(int kind, long double rho, long double sigma, long double *q,
  * :253-286 (DIFFUSION), :451-469 (displacement, rho, sigma, kernel). */
 typedef struct {
     double dr_t[3], dr_s[3], da[3];
+    /* da split into its stretching (VORSTRETCH) and diffusion (DIFFUSION)
+     * parts; st + df rounds to da exactly, since da is formed as
+     * ((0 + crossed) + stretch) + dva */
+    double st[3], df[3];
 } pair_out;
 
 static inline void pair_eval(int kind, double nu, double s_src, double s_trg,
@@ -159,6 +163,7 @@ static inline void pair_eval(int kind, double nu, double s_src, double s_trg,
     for (int i = 0; i < 3; ++i) o->da[i] = 0.0;
     for (int i = 0; i < 3; ++i) o->da[i] += crossed[i];
     for (int i = 0; i < 3; ++i) o->da[i] += stretch[i];
+    for (int i = 0; i < 3; ++i) o->st[i] = o->da[i];
     /* DIFFUSION: dva = (v_trg*a_src - v_src*a_trg) * (Z*nu/(0.5*sigma*sigma)) */
     double sig12 = 0.5 * sigma * sigma;
     double k = Z * nu / sig12;
@@ -171,6 +176,7 @@ static inline void pair_eval(int kind, double nu, double s_src, double s_trg,
         dva[i] = k * dva[i];
     }
     for (int i = 0; i < 3; ++i) o->da[i] += dva[i];
+    for (int i = 0; i < 3; ++i) o->df[i] = dva[i];
 }
 
 /* --- heap-temporary emulation of the reference's per-pair gsl_vector use ---
@@ -361,41 +367,49 @@ static void omp_row(int faithful, int kind, double nu, size_t is, size_t tb, siz
     dvort[3 * is + 2] = qz + dvort[3 * is + 2];
 }
 
-/* One-sided per-target sum in long double: for every target i,
+/* One-sided per-target sum in long double: for target i,
  *   u_i  = sum_{j!=i} q (a_j x d)
  *   da_i = sum_{j!=i} q (a_i x a_j) + F (d.(a_i x a_j)) d
  *                     + (nu Z/(sigma^2/2)) (v_i a_j - v_j a_i),  d = x_i - x_j,
  * which is what the symmetric sweep adds up (the pair terms are odd under
  * exchange of target and source). */
+static void onesided_ld_target(int kind, double nu, size_t n, const double *pos,
+                               const double *vort, const double *rad, const double *vol,
+                               size_t i, long double u[3], long double st[3],
+                               long double df[3]) {
+    for (int m = 0; m < 3; ++m) u[m] = st[m] = df[m] = 0.0L;
+    const double *xi = pos + 3 * i, *ai = vort + 3 * i;
+    for (size_t j = 0; j < n; ++j) {
+        if (j == i) continue;
+        const double *xj = pos + 3 * j, *aj = vort + 3 * j;
+        long double d[3] = {(long double)xi[0] - xj[0], (long double)xi[1] - xj[1],
+                            (long double)xi[2] - xj[2]};
+        long double rho = sqrtl(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        long double sig2 = 0.5L * ((long double)rad[i] * rad[i] + (long double)rad[j] * rad[j]);
+        long double q, F, Z;
+        orc_kernel_ld(kind, rho, sqrtl(sig2), &q, &F, &Z);
+        long double c[3] = {aj[1] * d[2] - aj[2] * d[1], aj[2] * d[0] - aj[0] * d[2],
+                            aj[0] * d[1] - aj[1] * d[0]};
+        long double e[3] = {(long double)ai[1] * aj[2] - (long double)ai[2] * aj[1],
+                            (long double)ai[2] * aj[0] - (long double)ai[0] * aj[2],
+                            (long double)ai[0] * aj[1] - (long double)ai[1] * aj[0]};
+        long double de = d[0] * e[0] + d[1] * e[1] + d[2] * e[2];
+        long double k = (long double)nu * Z / (0.5L * sig2);
+        for (int m = 0; m < 3; ++m) {
+            u[m] += q * c[m];
+            st[m] += q * e[m] + F * de * d[m];
+            df[m] += k * ((long double)vol[i] * aj[m] - (long double)vol[j] * ai[m]);
+        }
+    }
+}
+
 static void onesided_ld(int kind, double nu, size_t n, const double *pos, const double *vort,
                         const double *rad, const double *vol, double *vel, double *stretch,
                         double *diff) {
 #pragma omp parallel for schedule(dynamic, 16)
     for (size_t i = 0; i < n; ++i) {
-        long double u[3] = {0, 0, 0}, st[3] = {0, 0, 0}, df[3] = {0, 0, 0};
-        const double *xi = pos + 3 * i, *ai = vort + 3 * i;
-        for (size_t j = 0; j < n; ++j) {
-            if (j == i) continue;
-            const double *xj = pos + 3 * j, *aj = vort + 3 * j;
-            long double d[3] = {(long double)xi[0] - xj[0], (long double)xi[1] - xj[1],
-                                (long double)xi[2] - xj[2]};
-            long double rho = sqrtl(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-            long double sig2 = 0.5L * ((long double)rad[i] * rad[i] + (long double)rad[j] * rad[j]);
-            long double q, F, Z;
-            orc_kernel_ld(kind, rho, sqrtl(sig2), &q, &F, &Z);
-            long double c[3] = {aj[1] * d[2] - aj[2] * d[1], aj[2] * d[0] - aj[0] * d[2],
-                                aj[0] * d[1] - aj[1] * d[0]};
-            long double e[3] = {(long double)ai[1] * aj[2] - (long double)ai[2] * aj[1],
-                                (long double)ai[2] * aj[0] - (long double)ai[0] * aj[2],
-                                (long double)ai[0] * aj[1] - (long double)ai[1] * aj[0]};
-            long double de = d[0] * e[0] + d[1] * e[1] + d[2] * e[2];
-            long double k = (long double)nu * Z / (0.5L * sig2);
-            for (int m = 0; m < 3; ++m) {
-                u[m] += q * c[m];
-                st[m] += q * e[m] + F * de * d[m];
-                df[m] += k * ((long double)vol[i] * aj[m] - (long double)vol[j] * ai[m]);
-            }
-        }
+        long double u[3], st[3], df[3];
+        onesided_ld_target(kind, nu, n, pos, vort, rad, vol, i, u, st, df);
         for (int m = 0; m < 3; ++m) {
             vel[3 * i + m] = (double)u[m];
             if (diff) {
@@ -464,6 +478,66 @@ int orc_rates_split_ld(int kind, double nu, size_t n, const double *pos, const d
                        double *stretch, double *diffusion) {
     if (kind != ORC_KERNEL_HIGHORDER && kind != ORC_KERNEL_GAUSSIAN) return -1;
     onesided_ld(kind, nu, n, pos, vort, radius, volume, vel, stretch, diffusion);
+    return 0;
+}
+
+/* Rates of a sample of targets against every particle.  interact() pairs
+ * every particle with every other once (interaction.cpp:95-108: each wake with
+ * itself, then each pair of wakes) and adds each pair's terms to both
+ * particles (interaction_utils.h:429-481, parallel.cpp:13-85); the terms a
+ * particle receives as the pair's source are the exact negations of those it
+ * would receive as its target (every term is odd under exchange: the
+ * displacement and a_t x a_s change sign, sigma and the kernel do not), so a
+ * particle's rate is the sum over j != i of its target-side terms.  Here that
+ * sum runs over j in index order, with pair_eval's binary64 arithmetic (ld ==
+ * 0), or in long double (ld != 0, onesided_ld_target). */
+int orc_rates_sample(int ld, int kind, double nu, size_t n, const double *pos,
+                     const double *vort, const double *radius, const double *volume,
+                     const double *uinf, size_t ns, const size_t *targets, double *vel,
+                     double *stretch, double *diffusion) {
+    if (kind != ORC_KERNEL_HIGHORDER && kind != ORC_KERNEL_GAUSSIAN) return -1;
+    if (!targets || !vel || !stretch) return -1;
+    for (size_t k = 0; k < ns; ++k)
+        if (targets[k] >= n) return -1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t k = 0; k < ns; ++k) {
+        const size_t i = targets[k];
+        double u[3] = {0.0, 0.0, 0.0}, st[3] = {0.0, 0.0, 0.0}, df[3] = {0.0, 0.0, 0.0};
+        if (ld) {
+            long double lu[3], ls[3], lf[3];
+            onesided_ld_target(kind, nu, n, pos, vort, radius, volume, i, lu, ls, lf);
+            for (int m = 0; m < 3; ++m) {
+                u[m] = (double)lu[m];
+                if (diffusion) {
+                    st[m] = (double)ls[m];
+                    df[m] = (double)lf[m];
+                } else {
+                    st[m] = (double)(ls[m] + lf[m]);
+                }
+            }
+        } else {
+            for (size_t j = 0; j < n; ++j) {
+                if (j == i) continue;
+                pair_out o;
+                pair_eval(kind, nu, radius[j], radius[i], pos + 3 * j, pos + 3 * i,
+                          vort + 3 * j, vort + 3 * i, volume[j], volume[i], &o);
+                for (int m = 0; m < 3; ++m) {
+                    u[m] += o.dr_t[m];
+                    if (diffusion) {
+                        st[m] += o.st[m];
+                        df[m] += o.df[m];
+                    } else {
+                        st[m] += o.da[m];
+                    }
+                }
+            }
+        }
+        for (int m = 0; m < 3; ++m) {
+            vel[3 * k + m] = uinf ? u[m] + uinf[m] : u[m];
+            stretch[3 * k + m] = st[m];
+            if (diffusion) diffusion[3 * k + m] = df[m];
+        }
+    }
     return 0;
 }
 

@@ -78,6 +78,18 @@ int orc_rates_split_ld(int kind, double nu, size_t n, const double *pos, const d
                        const double *radius, const double *volume, double *vel,
                        double *stretch, double *diffusion);
 
+/* Rates of the targets targets[0..ns) against all n particles (every wake
+ * concatenated: interact() pairs all of them), N x 3 rows k = 0..ns-1 in
+ * vel / stretch / diffusion; diffusion == NULL puts stretching + diffusion in
+ * stretch.  ld == 0: the reference's binary64 pair arithmetic, summed over
+ * sources in index order; ld != 0: long double.  uinf (may be NULL) is added
+ * to vel.  For checking a GPU pass at sizes where all N rates would take too
+ * long on the CPU.  Returns 0, or -1 on bad arguments. */
+int orc_rates_sample(int ld, int kind, double nu, size_t n, const double *pos,
+                     const double *vort, const double *radius, const double *volume,
+                     const double *uinf, size_t ns, const size_t *targets, double *vel,
+                     double *stretch, double *diffusion);
+
 /* State vector <-> particle arrays in the reference layout: for each wake in
  * turn, [positions 3n | vorticities 3n] (wake.cpp:87-120). */
 void orc_pack_state(size_t nwake, const size_t *wake_n, const double *pos,

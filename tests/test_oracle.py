@@ -415,3 +415,142 @@ def test_enstrophy_helicity_symmetry():
     R = gen.rotation_from_uniforms(0.2, 0.4, 0.9)
     assert math.isclose(orc.helicity(pos @ R.T, vort @ R.T, rad), H, rel_tol=1e-10, abs_tol=1e-18)
     assert math.isclose(orc.enstrophy(pos, vort, rad, wake_n=[10, 20]), E, rel_tol=1e-12)
+
+
+def test_angular_impulse_offset_ring_closed_form():
+    """calculateAngularImpulse (interaction.cpp:261-275), A = (1/3) sum x x (x x a),
+    on a planar ring of N points at radius R about centre c with
+    a = Gamma R dpsi e_psi: the sums over psi of the degree-2 trigonometric
+    terms are exact for N >= 3, giving A = pi R^2 Gamma (c_y, -c_x, 0) = c x I
+    (I = pi R^2 Gamma z, the linear impulse); c_z drops out."""
+    for c in ((0.0, 0.0, 0.0), (0.4, -0.7, 0.25), (-1.5, 0.3, -2.0)):
+        w = gen.vortex_ring(48, 1, R=1.3, core=0.1, gamma=0.8, jitter=0.0, center=c)
+        dg = orc.diagnostics(w["pos"], w["vort"], w["radius"], ke=False)
+        I = math.pi * 1.3 ** 2 * 0.8
+        want = np.array([I * c[1], -I * c[0], 0.0])
+        assert np.allclose(dg["angular_impulse"], want, rtol=1e-13, atol=1e-14)
+        assert np.allclose(np.cross(c, dg["linear_impulse"]), want, rtol=1e-13, atol=1e-14)
+
+
+def test_enstrophy_self_and_coincident_pair():
+    """ENST (highorder.h:69-86) with ENSTROPHY (interaction_utils.h:107-156):
+    one particle gives 5 |a|^2 / (4 pi s^3); at zero separation with equal
+    cores the pair factor F1 is the same 5/(4 pi s^3) (F2 = 0), so two
+    coincident particles have the enstrophy of one particle carrying a1 + a2
+    (calculateEnstrophy, interaction.cpp:277-298, doubles each pair)."""
+    rng = np.random.default_rng(21)
+    a1, a2 = rng.normal(size=3), rng.normal(size=3)
+    for s in (0.02, 0.3, 1.7):
+        e1 = orc.enstrophy(np.zeros((1, 3)), a1[None, :], np.array([s]))
+        assert math.isclose(e1, 5.0 * (a1 @ a1) / (4 * math.pi * s ** 3), rel_tol=1e-14)
+        pos = np.full((2, 3), 0.37)
+        e2 = orc.enstrophy(pos, np.stack([a1, a2]), np.array([s, s]))
+        a = a1 + a2
+        assert math.isclose(e2, 5.0 * (a @ a) / (4 * math.pi * s ** 3), rel_tol=1e-13)
+
+
+def test_enstrophy_pair_far_field_and_unit_separation():
+    """ENST pair factors: far field (rho >> sigma) F1 -> -1/(4 pi rho^3) and
+    F2 -> -15/(8 pi rho^5) (relative corrections O(sigma^2/rho^2)); at
+    rho = sigma exactly, F1 = 1/(8 pi 2^3.5 sigma^3) (10 - 2 - 7 = 1) and
+    F2 = -15/(8 pi 2^3.5 sigma^5).  E(pair) - self terms = 2 (F1 a1.a2 +
+    F2 (a1.x12)(a2.x12))."""
+    rng = np.random.default_rng(22)
+    a1, a2 = rng.normal(size=3), rng.normal(size=3)
+    s1, s2 = 0.05, 0.09
+    sig = math.sqrt(0.5 * (s1 * s1 + s2 * s2))
+    selfs = 5.0 * (a1 @ a1 / s1 ** 3 + a2 @ a2 / s2 ** 3) / (4 * math.pi)
+    for rho, F1, F2, tol in ((1e4 * sig, -1 / (4 * math.pi * (1e4 * sig) ** 3),
+                              -15 / (8 * math.pi * (1e4 * sig) ** 5), 1e-6),
+                             (sig, 1 / (8 * math.pi * 2 ** 3.5 * sig ** 3),
+                              -15 / (8 * math.pi * 2 ** 3.5 * sig ** 5), 1e-12)):
+        d = rng.normal(size=3)
+        d *= rho / np.linalg.norm(d)
+        x1 = np.array([0.2, -0.1, 0.3])
+        pos = np.stack([x1, x1 + d])
+        e = orc.enstrophy(pos, np.stack([a1, a2]), np.array([s1, s2]))
+        want = 2.0 * (F1 * (a1 @ a2) + F2 * (a1 @ d) * (a2 @ d))
+        assert math.isclose(e - selfs, want, rel_tol=tol, abs_tol=1e-12 * abs(selfs)), (rho, e - selfs, want)
+
+
+def test_helicity_two_particles_and_velocity_identity():
+    """HELICITY (interaction_utils.h:167-190): H = 2 sum_{I<J} q(rho, sigma)
+    x_IJ.(a_I x a_J) with x_IJ = x_J - x_I (calculateHelicity,
+    interaction.cpp:317-332).  Hand case: x12 = (0, 0, rho), a1 = x, a2 = y
+    gives H = 2 q rho > 0.  In general, since u_i = sum_j q (a_j x (x_i - x_j))
+    (VELOCITY, interaction_utils.h:52-58), sum_i a_i.u_i = 2 sum_{I<J} q
+    (x_I - x_J).(a_I x a_J): the reference's helicity is minus the sum of
+    a_i.u_i over its own (pinned) velocities."""
+    s, rho = 0.1, 0.07
+    pos = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, rho]])
+    vort = np.array([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    q, _, _ = orc.kernel(rho, s)
+    H = orc.helicity(pos, vort, np.array([s, s]))
+    assert q > 0 and math.isclose(H, 2 * q * rho, rel_tol=1e-14)
+    # exchanging the particles' labels leaves it unchanged
+    assert math.isclose(orc.helicity(pos[::-1], vort[::-1], np.array([s, s])), H, rel_tol=1e-14)
+    # mirror image (z -> -z, axial vectors pick up det = -1): sign flips
+    assert math.isclose(orc.helicity(pos * [1, 1, -1], -(vort * [1, 1, -1]), np.array([s, s])),
+                        -H, rel_tol=1e-14)
+    for n, seed in ((2, 30), (17, 31), (60, 32)):
+        p, a, rad, vol = _random_system(n, seed)
+        v, _ = orc.rates(p, a, rad, vol, 0.0, orc.ONESIDED_LD)
+        H = orc.helicity(p, a, rad)
+        want = -float(np.sum(a * v))
+        assert math.isclose(H, want, rel_tol=1e-11, abs_tol=1e-14 * float(np.sum(np.abs(a * v)))), (n, H, want)
+
+
+# ------------------------------------------------------------ sampled rates
+def test_rates_sample_matches_full_oracle():
+    """orc_rates_sample (targets against every particle, the reference's
+    binary64 pair arithmetic summed in source order) equals the symmetric
+    serial sweep (interaction.cpp:95-108) and the OpenMP sweep to summation
+    order, with several wakes concatenated; the long-double form equals the
+    one-sided long-double sums exactly (same arithmetic, same order); the
+    stretching and diffusion split adds up to the total; a subset returns the
+    same rows bit for bit; the freestream is added to velocities only."""
+    a = gen.vortex_ring(24, 16, R=1.0, core=0.1, seed=41)
+    b = gen.vortex_ring(20, 9, R=0.8, core=0.1, seed=42, center=(0, 0, 0.3))
+    w = {k: np.concatenate([a[k], b[k]]) for k in ("pos", "vort", "radius", "volume")}
+    n = w["radius"].size
+    nu = 1e-3
+    args = (w["pos"], w["vort"], w["radius"], w["volume"])
+    rv, rd = orc.rates(*args, nu, mode=orc.SERIAL, wake_n=[a["n"], b["n"]])
+    ov, od = orc.rates(*args, nu, mode=orc.OMP_FAITHFUL, wake_n=[a["n"], b["n"]])
+    allt = np.arange(n)
+    v, st, df = orc.rates_sample(*args, nu, allt)
+    assert per_particle_rel(v, rv) < 1e-13 and per_particle_rel(st + df, rd) < 1e-13
+    assert per_particle_rel(v, ov) < 1e-13 and per_particle_rel(st + df, od) < 1e-13
+    v2, d2 = orc.rates_sample(*args, nu, allt, split=False)
+    assert np.array_equal(v2, v) and per_particle_rel(d2, st + df) < 1e-12
+    _, st0, df0 = orc.rates_sample(*args, 0.0, allt)
+    assert np.array_equal(st0, st) and np.all(df0 == 0.0)
+    lv, ls, ld = orc.rates_sample(*args, nu, allt, ld=True)
+    sv, ss, sd = orc.rates_split(*args, nu)
+    assert np.array_equal(lv, sv) and np.array_equal(ls, ss) and np.array_equal(ld, sd)
+    assert per_particle_rel(v, lv) < 1e-13 and per_particle_rel(st, ls) < 1e-11
+    sub = np.array([n - 1, 0, 17, 300, 301, 5])
+    vs, ss_, ds = orc.rates_sample(*args, nu, sub)
+    assert np.array_equal(vs, v[sub]) and np.array_equal(ss_, st[sub]) and np.array_equal(ds, df[sub])
+    u = np.array([0.01, -0.02, 0.05])
+    vu, su, du = orc.rates_sample(*args, nu, sub, uinf=u)
+    assert np.array_equal(vu, vs + u) and np.array_equal(su, ss_) and np.array_equal(du, ds)
+    with pytest.raises(ValueError):
+        orc.rates_sample(*args, nu, [n])
+
+
+def test_rates_sample_gaussian_kernel():
+    pos, vort, rad, vol = _random_system(40, 43)
+    v, d = orc.rates(pos, vort, rad, vol, 0.01, orc.SERIAL, orc.GAUSSIAN)
+    sv, sst, sdf = orc.rates_sample(pos, vort, rad, vol, 0.01, np.arange(40), kind=orc.GAUSSIAN)
+    assert per_particle_rel(sv, v) < 1e-13 and per_particle_rel(sst + sdf, d) < 1e-13
+
+
+def test_sample_targets_cover_edges():
+    from helpers import sample_targets
+    t = sample_targets(262144, 512)
+    assert t[0] == 0 and t[-1] == 262143 and np.all(np.diff(t) > 0)
+    for e in (63, 64, 255, 256, 511, 512, 16383, 16384):
+        assert e in t
+    # every stratum holds a pick
+    assert np.all(np.histogram(t, bins=512, range=(0, 262144))[0] >= 1)
