@@ -251,3 +251,34 @@ def checksum(w: dict) -> dict:
         a = np.asarray(w[k], dtype=np.float64)
         out[k] = [float(a.sum()), float(np.abs(a).sum()), float((a * a).sum())]
     return out
+
+
+def serving_stream(count: int, seed: int = 1, rate_hz: float = 1.0, n_lo: int = 2048,
+                   n_hi: int = 262144, s_lo: int = 50, s_hi: int = 500,
+                   rotor_dt: float = 1e-5) -> list:
+    """Jobs of the serving workload, in arrival order: Poisson arrivals at
+    rate_hz (arrival_ms), each a vortex ring (R = 1, core 0.1, circulation 1,
+    nu = 1e-3, dt = 0.005) or a frozen rotor wake (nu = 1e-4, freestream 0.05
+    along -z, dt = rotor_dt), 50/50; N log-uniform in [n_lo, n_hi] (a ring
+    is rounded to n_azi x n_cs, a rotor wake to whole shedding rows of 256);
+    steps uniform in [s_lo, s_hi]; precision binary64 or compensated binary32
+    50/50 (0 / 1).  Job k draws five uniforms from the SplitMix64 stream of
+    ``seed``."""
+    u = uniform(seed, 5 * count).reshape(count, 5)
+    jobs, t = [], 0.0
+    for k in range(count):
+        a = u[k]
+        n = int(n_lo * (n_hi / n_lo) ** a[0])
+        steps = int(s_lo + (s_hi - s_lo) * a[1])
+        prec = 0 if a[2] < 0.5 else 1
+        if a[3] < 0.5:
+            n_cs = ring_cs_for(n, 1.0, 0.1)
+            w = vortex_ring(max(8, n // n_cs), n_cs, R=1.0, core=0.1, seed=seed + k)
+            job = dict(kind="ring", w=w, nu=1e-3, dt=0.005, uinf=(0.0, 0.0, 0.0))
+        else:
+            w = rotor_wake(max(1, n // 256))
+            job = dict(kind="rotor", w=w, nu=1e-4, dt=rotor_dt, uinf=(0.0, 0.0, -0.05))
+        job.update(n=int(w["n"]), steps=steps, precision=prec, arrival_ms=t)
+        jobs.append(job)
+        t += -math.log(1.0 - a[4]) / rate_hz * 1000.0
+    return jobs
