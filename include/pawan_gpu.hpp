@@ -30,10 +30,20 @@
 // (resolve.cpp:10-57) runs on the device-backed interaction unchanged.  RK4 steps run entirely on
 // the device (pw_rk4_steps); the integrator's state vector is refreshed after
 // each step, as __rk4::step leaves it.
+//
+// Lifetime.  pawan's __system and __interaction destructors are not virtual
+// (system.h:34, interaction.h:175), so `delete S` through an __interaction*
+// never runs this adapter's destructor.  Delete the adapter through its own
+// type (`auto *S = new __gpu_interaction(W, o); ... delete S;`), or call
+// pawan_gpu::release(S) before deleting it through the base.  Every device
+// context the adapter creates is also listed in a process-wide registry that
+// destroys the ones still alive when the program exits.
 #pragma once
 
 #include <cstddef>
 #include <cstdio>
+#include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -48,6 +58,46 @@ inline void check(int rc, const pw_ctx *ctx) {
                                  std::to_string(rc) + ")");
 }
 
+namespace detail {
+// Device contexts created by adapters and not yet destroyed.  Constructed by
+// the first adapter after its pw_create returns, so that it is destroyed at
+// exit before the HIP runtime state that pw_create brought up; its destructor
+// destroys the contexts still listed (adapters deleted through a base pointer
+// or never deleted).
+struct registry {
+    std::mutex m;
+    std::set<pw_ctx *> live;
+    ~registry() {
+        for (pw_ctx *c : live) pw_destroy(c);
+    }
+};
+inline registry &contexts() {
+    static registry r;
+    return r;
+}
+inline void track(pw_ctx *c) {
+    registry &r = contexts();
+    std::lock_guard<std::mutex> g(r.m);
+    r.live.insert(c);
+}
+inline void untrack_and_destroy(pw_ctx *c) {
+    if (!c) return;
+    {
+        registry &r = contexts();
+        std::lock_guard<std::mutex> g(r.m);
+        r.live.erase(c);
+    }
+    pw_destroy(c);
+}
+}  // namespace detail
+
+// device contexts created by adapters and still alive
+inline size_t live_contexts() {
+    detail::registry &r = detail::contexts();
+    std::lock_guard<std::mutex> g(r.m);
+    return r.live.size();
+}
+
 // Common base of device-backed systems, so an integrator can reach the context
 // through the reference's __system pointer.
 class device_system {
@@ -56,7 +106,18 @@ class device_system {
     virtual pw_ctx *context() = 0;
     // copy device positions, vorticities and last rates into the wakes
     virtual void pull() = 0;
+    // destroy the device context now (its stream and device memory); later
+    // calls on the system fail with PW_EINVAL.  Idempotent.
+    virtual void release() = 0;
 };
+
+// Release the device context of a pawan_gpu system held through any of its
+// bases (e.g. pawan::__interaction *), before a `delete` through that base,
+// which cannot reach the adapter's destructor.  No effect on other systems.
+template <class System>
+void release(System *S) {
+    if (device_system *d = dynamic_cast<device_system *>(S)) d->release();
+}
 
 struct options {
     double nu = 2.5e-3;  // reference default, interaction.cpp:13
@@ -78,6 +139,7 @@ class interaction : public Interaction, public device_system {
         c.device = o.device;
         for (int k = 0; k < 3; ++k) c.uinf[k] = o.uinf[k];
         check(pw_create(&ctx_, &c), nullptr);
+        detail::track(ctx_);
         this->_nu = o.nu;
     }
     // __interaction(W) and __interaction(W1, W2), interaction.cpp:20-27
@@ -86,11 +148,15 @@ class interaction : public Interaction, public device_system {
         addWake(W1);
         addWake(W2);
     }
-    ~interaction() override { pw_destroy(ctx_); }
+    ~interaction() override { release(); }
     interaction(const interaction &) = delete;
     interaction &operator=(const interaction &) = delete;
 
     pw_ctx *context() override { return ctx_; }
+    void release() override {
+        detail::untrack_and_destroy(ctx_);
+        ctx_ = nullptr;
+    }
 
     // interaction.cpp:35-39, plus a copy of the particles to the device
     void addWake(Wake *W) override {

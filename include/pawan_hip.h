@@ -189,6 +189,13 @@ int pw_write_wait(pw_ctx *ctx);
 int pw_append_particles(pw_ctx *ctx, size_t wake, size_t n, const double *pos, const double *vort,
                         const double *radius, const double *volume);
 
+/* The pair-pass variant the context runs for its precision (builds the
+ * device job if needed): threads per workgroup, targets per thread, unroll
+ * (0 on the symmetric pass) and PW_SCHEME_ONESIDED / PW_SCHEME_SYMMETRIC.
+ * No reference counterpart: reports what pw_config's zero fields resolved to
+ * (defaults, the tuning cache). */
+int pw_get_variant(pw_ctx *ctx, pw_variant *out);
+
 /* device time of the last pw_solve / pw_rk4_steps / pw_euler_steps call,
  * measured with HIP events on the context's stream: total ms and the summed
  * ms of the pair-interaction kernel alone, plus the pairs it evaluated */

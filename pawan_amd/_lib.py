@@ -78,6 +78,7 @@ SIGNATURES = {
     "pw_euler_steps": (C.c_int, [C.c_void_p, C.c_double, C.c_size_t]),
     "pw_diagnostics": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "pw_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "pw_get_variant": (C.c_int, [C.c_void_p, C.POINTER(PwVariant)]),
     "pw_diagnose": (C.c_int, [C.c_void_p, C.POINTER(PwDiag)]),
     "pw_resolve": (C.c_int, [C.c_void_p, _dp]),
     "pw_write": (C.c_int, [C.c_void_p, C.c_void_p]),

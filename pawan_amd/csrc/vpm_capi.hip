@@ -52,9 +52,12 @@ int guarded(std::string *err, F &&f) {
 // 1 target, which halves them as 256 x 2 does with the faster one-target pass
 // (973 against 1,052 ms per RHS at N = 1,048,576).  From 131,072 to 524,288
 // 512 x 1 and 256 x 1 are within 1% of each other (the autotuner picks).
-int default_block(size_t n, int precision, int kernel, int scheme) {
+// An explicit two targets per thread, or a chunk length that 512 does not
+// divide, keeps 256 threads there (the one 512-thread variant takes one
+// target per thread; chunks are multiples of the block).
+int default_block(size_t n, int precision, int kernel, int scheme, int tpt = 0, int chunk = 0) {
     if (precision == PW_FP32C && kernel == PW_KERNEL_HIGHORDER && scheme != PW_SCHEME_ONESIDED &&
-        n > 524288)
+        n > 524288 && tpt != 2 && chunk % 512 == 0)
         return 512;
     if (n >= 32768) return 256;
     if (n >= 8192) return 128;
@@ -93,11 +96,14 @@ struct pw_ctx {
 
     size_t total() const { return (size_t)woff.back(); }
 
-    pw::Variant choose_variant() const {
+    // the variant for a context of n particles (default: the current total)
+    pw::Variant choose_variant() const { return choose_variant(total()); }
+    pw::Variant choose_variant(size_t n) const {
         pw::Variant v;
         v.block = cfg.block ? cfg.block
-                            : default_block(total(), cfg.precision, cfg.kernel, cfg.scheme);
-        v.tpt = cfg.targets_per_thread ? cfg.targets_per_thread : default_tpt(total(), cfg.precision, cfg.kernel);
+                            : default_block(n, cfg.precision, cfg.kernel, cfg.scheme,
+                                            cfg.targets_per_thread, cfg.chunk);
+        v.tpt = cfg.targets_per_thread ? cfg.targets_per_thread : default_tpt(n, cfg.precision, cfg.kernel);
         if (v.block == 512) v.tpt = 1;  // the one 512-thread variant
         if (cfg.unroll) v.unroll64 = v.unroll32 = cfg.unroll;
         v.sym32 = cfg.scheme != PW_SCHEME_ONESIDED;
@@ -105,7 +111,7 @@ struct pw_ctx {
         const char *cache = std::getenv("PW_TUNE_CACHE");
         if (cache && !cfg.block && !cfg.targets_per_thread && !cfg.unroll && !cfg.scheme) {
             pw_variant t;
-            if (pw_tune_lookup(cache, cfg.kernel, cfg.precision, total(), &t) == PW_OK) {
+            if (pw_tune_lookup(cache, cfg.kernel, cfg.precision, n, &t) == PW_OK) {
                 v.block = t.block;
                 v.tpt = t.targets_per_thread;
                 if (cfg.precision == PW_FP64) {
@@ -120,15 +126,21 @@ struct pw_ctx {
         return v;
     }
 
+    // the engine runs variant v already
+    bool engine_runs(const pw::Variant &v) const {
+        if (!eng) return false;
+        const pw::Variant cur = eng->variant();
+        const int cur32 = cur.tpt32 ? cur.tpt32 : cur.tpt, v32 = v.tpt32 ? v.tpt32 : v.tpt;
+        return cur.block == v.block && cur.tpt == v.tpt && cur32 == v32 &&
+               cur.unroll64 == v.unroll64 && cur.unroll32 == v.unroll32 && cur.sym32 == v.sym32 &&
+               cur.sym64 == v.sym64;
+    }
+
     void rebuild() {
         if (total() == 0) throw Error(PW_EINVAL, "no particles: add a wake first");
         pw::Variant v = choose_variant();
         if (writer) writer->quiesce();  // queued frames read the current device state
-        pw::Variant cur = eng ? eng->variant() : pw::Variant{};
-        const int cur32 = cur.tpt32 ? cur.tpt32 : cur.tpt, v32 = v.tpt32 ? v.tpt32 : v.tpt;
-        if (!eng || cur.block != v.block || cur.tpt != v.tpt || cur32 != v32 ||
-            cur.unroll64 != v.unroll64 ||
-            cur.unroll32 != v.unroll32 || cur.sym32 != v.sym32 || cur.sym64 != v.sym64) {
+        if (!engine_runs(v)) {
             job = nullptr;
             eng.reset();
             eng.reset(new Engine(cfg.device, cfg.kernel, v));
@@ -307,14 +319,12 @@ int pw_get_wake(pw_ctx *ctx, size_t wake, double *pos, double *vort, double *vel
     return guarded(&ctx->err, [&] {
         if (wake >= ctx->wake_n.size()) throw Error(PW_EINVAL, "no such wake");
         ctx->ready();
-        std::vector<double> tmp(3 * ctx->total());
         double *outs[4] = {pos, vort, vel, dvort};
-        const size_t o = 3 * (size_t)ctx->woff[wake], m = 3 * ctx->wake_n[wake];
-        for (int k = 0; k < 4; ++k) {
-            if (!outs[k]) continue;
-            ctx->eng->download_aos(ctx->job, k, tmp.data());
-            std::memcpy(outs[k], tmp.data() + o, m * sizeof(double));
-        }
+        // only this wake's particles cross to the host
+        for (int k = 0; k < 4; ++k)
+            if (outs[k])
+                ctx->eng->download_aos(ctx->job, k, outs[k], (size_t)ctx->woff[wake],
+                                       ctx->wake_n[wake]);
     });
 }
 
@@ -435,7 +445,11 @@ int pw_append_particles(pw_ctx *ctx, size_t wake, size_t n, const double *pos, c
         if (n == 0) return;
         if (!pos || !vort || !radius || !volume) throw Error(PW_EINVAL, "null array");
         const size_t last = ctx->wake_n.size() - 1;
-        if (ctx->job && !ctx->dirty && wake == last) {
+        // in place only while the grown context keeps its kernel variant (a
+        // binary32 context past 524,288 particles moves to 512 threads, which
+        // halves its source-side slots); otherwise the next call rebuilds it
+        if (ctx->job && !ctx->dirty && wake == last &&
+            ctx->engine_runs(ctx->choose_variant(ctx->total() + n))) {
             // the wake's particles end the device arrays: append in place
             ctx->eng->append(ctx->job, n, pos, vort, radius, volume);
             ctx->pos.insert(ctx->pos.end(), pos, pos + 3 * n);
@@ -455,6 +469,20 @@ int pw_append_particles(pw_ctx *ctx, size_t wake, size_t n, const double *pos, c
         ctx->vol.insert(ctx->vol.begin() + at, volume, volume + n);
         ctx->wake_n[wake] += n;
         for (size_t w = wake + 1; w < ctx->woff.size(); ++w) ctx->woff[w] += (int64_t)n;
+    });
+}
+
+int pw_get_variant(pw_ctx *ctx, pw_variant *out) {
+    if (!ctx || !out) return PW_EINVAL;
+    return guarded(&ctx->err, [&] {
+        ctx->ready();
+        const pw::Variant v = ctx->eng->variant();
+        const bool fp32 = ctx->cfg.precision == PW_FP32C;
+        out->block = pw::block_for(v.block, ctx->cfg.precision);
+        out->targets_per_thread = fp32 && v.tpt32 ? v.tpt32 : v.tpt;
+        const bool sym = fp32 ? v.sym32 != 0 : v.sym64 != 0;
+        out->unroll = sym ? 0 : (fp32 ? v.unroll32 : v.unroll64);
+        out->scheme = sym ? PW_SCHEME_SYMMETRIC : PW_SCHEME_ONESIDED;
     });
 }
 

@@ -754,19 +754,24 @@ void Engine::upload_state_soa(Job *j, const double *pos, const double *vort) {
     PW_HIP(hipStreamSynchronize(stream_));
 }
 
-void Engine::download_aos(Job *j, int which, double *out) {
+void Engine::download_aos(Job *j, int which, double *out) { download_aos(j, which, out, 0, j->n); }
+
+void Engine::download_aos(Job *j, int which, double *out, size_t first, size_t count) {
     PW_HIP(hipSetDevice(device_));
-    const size_t np = j->npad, n = j->n;
-    if (dstage_.bytes < 3 * n * sizeof(double)) {
+    const size_t np = j->npad;
+    if (first > j->n || count > j->n - first) throw Error(PW_EINVAL, "particle range out of bounds");
+    if (count == 0) return;
+    if (dstage_.bytes < 3 * count * sizeof(double)) {
         PW_HIP(hipStreamSynchronize(stream_));
         dstage_.reset(round_up(6 * np * sizeof(double), 4096));
     }
     JobDev d = j->dev();
     double *src = which == 0 ? d.x0 : which == 1 ? d.x0 + 3 * np : which == 2 ? d.rate : d.rate + 3 * np;
-    hipLaunchKernelGGL(aos_soa_kernel, dim3(grid_for(n)), dim3(256), 0, stream_,
-                       dstage_.as<double>(), src, (int)n, (int)np, 1);
+    hipLaunchKernelGGL(aos_soa_kernel, dim3(grid_for(count)), dim3(256), 0, stream_,
+                       dstage_.as<double>(), src + first, (int)count, (int)np, 1);
     PW_HIP(hipGetLastError());
-    PW_HIP(hipMemcpyAsync(out, dstage_.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    PW_HIP(hipMemcpyAsync(out, dstage_.p, 3 * count * sizeof(double), hipMemcpyDeviceToHost,
+                          stream_));
     PW_HIP(hipStreamSynchronize(stream_));
 }
 

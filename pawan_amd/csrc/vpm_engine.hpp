@@ -154,6 +154,8 @@ class Engine {
     // Host <-> device transfers (synchronous on the engine stream).
     void upload_state_soa(Job *j, const double *pos, const double *vort);  // N x 3 each
     void download_aos(Job *j, int which, double *out);  // which: 0 pos 1 vort 2 vel 3 dvort
+    // particles [first, first + count) only
+    void download_aos(Job *j, int which, double *out, size_t first, size_t count);
     // Queue a copy of the job's positions then vorticities (N x 3 each,
     // row-major) into device memory dst (6N doubles) on the engine stream,
     // after every step already queued; returns without waiting.

@@ -223,6 +223,13 @@ class Interaction:
         W._size = 6 * W._numParticles
         self._size = lib().pw_size(self._h)
 
+    def variant(self):
+        """The pair-pass variant this context runs (pw_get_variant)."""
+        v = _lib.PwVariant()
+        self._ck(lib().pw_get_variant(self._h, C.byref(v)))
+        return {"block": v.block, "targets_per_thread": v.targets_per_thread,
+                "unroll": v.unroll, "scheme": v.scheme}
+
     def last_timing(self):
         t, p, n = C.c_double(), C.c_double(), C.c_double()
         dp = C.POINTER(C.c_double)
@@ -375,6 +382,13 @@ class Batch:
 
     def status(self, jid):
         return lib().pw_batch_status(self._h, jid)
+
+    def variant(self):
+        """The pair-pass variant this context runs (pw_get_variant)."""
+        v = _lib.PwVariant()
+        self._ck(lib().pw_get_variant(self._h, C.byref(v)))
+        return {"block": v.block, "targets_per_thread": v.targets_per_thread,
+                "unroll": v.unroll, "scheme": v.scheme}
 
     def last_timing(self):
         t, p, n = C.c_double(), C.c_double(), C.c_double()

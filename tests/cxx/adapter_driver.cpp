@@ -55,8 +55,11 @@ int main(int argc, char **argv) {
     o.nu = 1e-3;
     int bad = 0;
     try {
-        gpu_interaction S(W1, W2, o);  // __interaction(W1, W2)
-        std::printf("driver: N = %zu + %zu, _size = %zu\n", n1, n2, S._size);
+        // pawan.cpp:30 with the adapter named by its own type, so that the
+        // `delete S` of pawan.cpp:67 reaches its destructor (pawan's
+        // __interaction destructor is not virtual, interaction.h:175)
+        auto *S = new gpu_interaction(W1, W2, o);  // __interaction(W1, W2)
+        std::printf("driver: N = %zu + %zu, _size = %zu\n", n1, n2, S->_size);
         // the oracle's view of the same particles
         std::vector<double> pos(3 * n), vort(3 * n), rad(n), vol(n);
         size_t wn[2] = {n1, n2};
@@ -76,7 +79,7 @@ int main(int argc, char **argv) {
         orc_rates(ORC_MODE_OMP_CLEAN, ORC_KERNEL_HIGHORDER, 1e-3, 2, wn, pos.data(), vort.data(),
                   rad.data(), vol.data(), nullptr, vel.data(), dv.data());
         // solve(): rates land in each wake's _velocity / _retvorcity
-        S.solve();
+        S->solve();
         std::vector<double> gv(W1->_velocity->data, W1->_velocity->data + 3 * n1),
             gd(W1->_retvorcity->data, W1->_retvorcity->data + 3 * n1);
         gv.insert(gv.end(), W2->_velocity->data, W2->_velocity->data + 3 * n2);
@@ -101,27 +104,31 @@ int main(int argc, char **argv) {
         // resolve(): each wake's _vorticityfield
         std::vector<double> fld(3 * n);
         orc_influence(ORC_KERNEL_HIGHORDER, 2, wn, pos.data(), vort.data(), rad.data(), fld.data());
-        S.resolve();
+        S->resolve();
         std::vector<double> gf(W1->_vorticityfield->data, W1->_vorticityfield->data + 3 * n1);
         gf.insert(gf.end(), W2->_vorticityfield->data, W2->_vorticityfield->data + 3 * n2);
         double ef = relerr(gf, fld);
         std::printf("driver: resolve() field vs oracle: %.3e\n", ef);
         bad |= !(ef < 1e-12);
         // getStates / getRates layout
-        mini::vec st(S._size), rt(S._size), ref(S._size);
-        S.getStates(&st);
+        mini::vec st(S->_size), rt(S->_size), ref(S->_size);
+        S->getStates(&st);
         orc_pack_state(2, wn, pos.data(), vort.data(), ref.data);
         bool same = true;
-        for (size_t i = 0; i < S._size; ++i) same = same && st.data[i] == ref.data[i];
+        for (size_t i = 0; i < S->_size; ++i) same = same && st.data[i] == ref.data[i];
         std::printf("driver: getStates layout %s\n", same ? "exact" : "WRONG");
         bad |= !same;
         // __rk4(t, n).integrate(S, IO): 3 steps of dt = 0.01, a frame per step
+        std::string base(path);
+        if (base.size() > 5 && base.compare(base.size() - 5, 5, ".wake") == 0)
+            base.resize(base.size() - 5);
+        mini::io IO("", base);  // IO->create_binary_file(".wake") opens `path`
         gpu_rk4 IN(0.03, 3);
-        IN.integrate(&S, path);
-        std::vector<double> x0(ref.data, ref.data + S._size), xg(S._size);
-        mini::vec fin(S._size);
-        S.getStates(&fin);
-        xg.assign(fin.data, fin.data + S._size);
+        IN.integrate(S, &IO);
+        std::vector<double> x0(ref.data, ref.data + S->_size), xg(S->_size);
+        mini::vec fin(S->_size);
+        S->getStates(&fin);
+        xg.assign(fin.data, fin.data + S->_size);
         orc_rk4(ORC_MODE_OMP_CLEAN, ORC_KERNEL_HIGHORDER, 1e-3, 2, wn, rad.data(), vol.data(),
                 nullptr, x0.data(), 0.01, 3);
         double ex = relerr(xg, x0);
@@ -209,8 +216,27 @@ int main(int argc, char **argv) {
             delete R;
         }
         double om[3], im[3], ke;
-        S.device_diagnostics(om, im, &ke);
+        S->device_diagnostics(om, im, &ke);
         std::printf("driver: kinetic energy %.12e\n", ke);
+        {  // a caller holding the adapter through pawan's base type releases it first
+            mini::interaction *B = new gpu_interaction(W1, o);
+            const size_t before = pawan_gpu::live_contexts();
+            pawan_gpu::release(B);
+            const size_t after = pawan_gpu::live_contexts();
+            bool threw = false;
+            try {
+                B->solve();
+            } catch (const std::exception &) {
+                threw = true;  // the context is gone
+            }
+            delete static_cast<gpu_interaction *>(B);
+            std::printf("driver: release() through the base: contexts %zu -> %zu, later solve() %s\n",
+                        before, after, threw ? "refused" : "RAN");
+            bad |= !(after + 1 == before && threw);
+        }
+        delete S;  // pawan.cpp:67
+        std::printf("driver: contexts alive after delete S: %zu\n", pawan_gpu::live_contexts());
+        bad |= pawan_gpu::live_contexts() != 0;
     } catch (const std::exception &e) {
         std::printf("driver: exception %s\n", e.what());
         bad = 1;

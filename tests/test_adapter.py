@@ -30,3 +30,139 @@ def test_adapter_driver_on_gpu(tmp_path):
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
     assert "driver: passed" in r.stdout
+
+
+# ------------------------------------------- the test double vs pawan's headers
+# pawan's own headers cannot be compiled here (they include GSL, which the
+# image lacks), so the test double the adapter is compiled and run against is
+# checked declaration by declaration against their text instead: every
+# virtual the double declares is declared by pawan's class with the same
+# parameter types (GSL and pawan class names mapped to the double's), access
+# and constness; every virtual the adapter overrides exists in both; the
+# destructors are virtual in the same classes; the wake members the adapter
+# reads have the same types.
+REF = "/root/reference/src"
+PAIRS = [("system/system.h", "__system", "system"),
+         ("interaction/interaction.h", "__interaction", "interaction"),
+         ("integration/integration.h", "__integration", "integration"),
+         ("integration/rk4.h", "__rk4", "rk4"),
+         ("wake/wake.h", "__wake", "wake"),
+         ("io/io.h", "__io", "io")]
+NAMES = {"gsl_vector": "vec", "gsl_matrix": "mat", "__system": "system", "__wake": "wake",
+         "__io": "io", "__interaction": "interaction", "__integration": "integration",
+         "__rk4": "rk4"}
+OVERRIDDEN = {"interaction": {"addWake", "solve", "resolve", "write", "diagnose", "setStates",
+                              "getRates", "getStates"},
+              "rk4": {"step"}, "integration": {"step"}}
+WAKE_MEMBERS = ["_size", "_numParticles", "_position", "_velocity", "_vorticity", "_retvorcity",
+                "_vorticityfield", "_radius", "_volume"]
+
+
+def _strip_comments(t):
+    import re
+    t = re.sub(r"/\*.*?\*/", " ", t, flags=re.S)
+    return re.sub(r"//[^\n]*", " ", t)
+
+
+def _class_body(text, name):
+    import re
+    m = re.search(r"\bclass\s+" + name + r"\b[^{;]*\{", text)
+    assert m, name
+    i, depth = m.end(), 1
+    j = i
+    while depth:
+        depth += {"{": 1, "}": -1}.get(text[j], 0)
+        j += 1
+    return text[i:j - 1]
+
+
+def _statements(body):
+    """Top-level statements of a class body, with the access each has;
+    inline function bodies end a statement."""
+    import re
+    out, cur, access, i = [], "", "private", 0
+    while i < len(body):
+        c = body[i]
+        if c == "{":
+            depth, j = 1, i + 1
+            while depth:
+                depth += {"{": 1, "}": -1}.get(body[j], 0)
+                j += 1
+            out.append((access, cur.strip()))
+            cur, i = "", j
+            continue
+        if c == ";":
+            out.append((access, cur.strip()))
+            cur = ""
+        else:
+            cur += c
+            m = re.fullmatch(r"\s*(public|protected|private)\s*:", cur)
+            if m:
+                access, cur = m.group(1), ""
+        i += 1
+    return [(a, s) for a, s in out if s]
+
+
+def _norm_type(t):
+    import re
+    t = t.split("=")[0].strip()
+    m = re.fullmatch(r"(.*?[\s\*&])(\w+)", t, re.S)  # drop a parameter's name
+    if m and m.group(1).strip() not in ("", "const"):
+        t = m.group(1)
+    for a, b in NAMES.items():
+        t = re.sub(r"\b" + a + r"\b", b, t)
+    return re.sub(r"\s+", "", t)
+
+
+def _decls(body):
+    """{(name, params): (access, virtual, const)} for methods; destructor
+    virtualness; {member: type} for data members."""
+    import re
+    meths, dtor_virtual, members = {}, False, {}
+    for access, st in _statements(body):
+        overrides = re.search(r"\boverride\b", st) is not None  # virtual by inheritance
+        st = re.sub(r"\boverride\b|\s*=\s*(default|0)\s*$", "", st).strip()
+        m = re.match(r"(virtual\s+)?(.*?)(~?\w+)\s*\(([^()]*)\)\s*(const)?\s*(:.*)?$", st, re.S)
+        if m:
+            name = m.group(3)
+            params = tuple(_norm_type(p) for p in m.group(4).split(",") if p.strip())
+            if name.startswith("~"):
+                dtor_virtual = bool(m.group(1))
+            else:
+                meths[(name, params)] = (access, bool(m.group(1)) or overrides, bool(m.group(5)))
+            continue
+        m = re.match(r"([\w:<>\s\*]+?)\s*(\*?)\s*(\w+)\s*(=.*)?$", st, re.S)
+        if m:
+            members[m.group(3)] = _norm_type(m.group(1) + m.group(2))
+    return meths, dtor_virtual, members
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="pawan's sources are not on this machine")
+def test_test_double_mirrors_reference_declarations():
+    mini = _strip_comments(open(os.path.join(CXX, "mini_pawan.hpp")).read())
+    for hdr, rname, mname in PAIRS:
+        ref = _strip_comments(open(os.path.join(REF, hdr)).read())
+        rm, rdv, rmem = _decls(_class_body(ref, rname))
+        mm, mdv, mmem = _decls(_class_body(mini, mname))
+        assert rdv == mdv, f"{rname}: destructor virtual {rdv}, double {mdv}"
+        for key, (acc, virt, const) in mm.items():
+            if not virt:
+                continue
+            assert key in rm, f"{mname}::{key} is not declared by {rname}"
+            assert rm[key] == (acc, virt, const), (mname, key, rm[key], (acc, virt, const))
+        for name in OVERRIDDEN.get(mname, ()):
+            rk = [k for k, v in rm.items() if k[0] == name and v[1]]
+            mk = [k for k, v in mm.items() if k[0] == name and v[1]]
+            assert rk and sorted(rk) == sorted(mk), (mname, name, rk, mk)
+        if mname == "wake":
+            for m in WAKE_MEMBERS:
+                assert rmem.get(m) == mmem.get(m), (m, rmem.get(m), mmem.get(m))
+        if mname == "interaction":
+            assert rmem.get("_nu") == mmem.get("_nu") == "double"
+    # the non-virtual integrate(__system *, __io *) and the constructors
+    ref = _strip_comments(open(os.path.join(REF, "integration/integration.h")).read())
+    rm, _, _ = _decls(_class_body(ref, "__integration"))
+    mm, _, _ = _decls(_class_body(mini, "integration"))
+    for key in (("integrate", ("system*", "io*")), ("__integration", ("constdouble&", "constsize_t&"))):
+        mkey = (key[0].replace("__", ""), key[1])
+        assert key in rm and mkey in mm, (key, sorted(rm), sorted(mm))

@@ -6,7 +6,7 @@ import pytest
 import oracle as orc
 import pawan_amd as pa
 from pawan_amd import generators as gen
-from helpers import per_particle_rel
+from helpers import per_particle_rel, rel_rms
 
 pytestmark = pytest.mark.gpu
 
@@ -264,3 +264,63 @@ def test_async_frame_write_failure_reported(tmp_path):
         S.write_async(g, 0.0)
     S.close()
     assert len(good.read_bytes()) == 8 + 8 + 8 + 56 * w["n"]
+
+
+# ------------------------------------------ binary32 variant past 524,288
+def _fp32c_rates(w, **kw):
+    S = pa.Interaction(pa.Wake.from_dict(w), nu=w["nu"], uinf=w["uinf"], precision=pa.FP32C, **kw)
+    S.solve()
+    r, v = S.getRates(), S.variant()
+    S.close()
+    return r, v
+
+
+def test_fp32c_large_context_explicit_tpt_and_chunk():
+    """Past 524,288 particles a binary32 symmetric context defaults to 512 x 1;
+    an explicit two targets per thread keeps 256 threads (256 x 2), and so
+    does a chunk length that 512 does not divide (here 129 x 256); each
+    solves, and the three agree to binary32 rounding."""
+    w = gen.config_rotor_wake(2112)  # 540,672 particles
+    print(f"case fp32c N={w['n']}: default / targets_per_thread 2 / chunk 33,024", flush=True)
+    r0, v0 = _fp32c_rates(w)
+    assert (v0["block"], v0["targets_per_thread"], v0["scheme"]) == (512, 1, pa.SCHEME_SYMMETRIC)
+    r2, v2 = _fp32c_rates(w, targets_per_thread=2)
+    assert (v2["block"], v2["targets_per_thread"]) == (256, 2), v2
+    rc, vc = _fp32c_rates(w, chunk=129 * 256)
+    assert vc["block"] == 256, vc
+    for r in (r2, rc):
+        e = rel_rms(r, r0)
+        print(f"  vs default: {e:.3e}", flush=True)
+        assert e < 1e-6
+
+
+def test_fp32c_append_across_524288_rebuilds_at_512():
+    """Particle emission that takes a binary32 context past 524,288 particles
+    moves it to the 512 x 1 pass (half the source-side slots) instead of
+    keeping the 256 x 1 pass chosen at its first build; appends below the
+    threshold stay in place.  The grown context's state is the particles
+    given, and its rates equal a context built at the final size, bit for bit."""
+    full = gen.config_rotor_wake(2050)  # 524,800 particles
+    n0, n1 = 522240, 522752
+    part = {k: full[k][:n0] for k in ("pos", "vort", "radius", "volume")}
+    print(f"case fp32c append {n0} -> {n1} -> {full['n']}", flush=True)
+    S = pa.Interaction(pa.Wake.from_dict(part), nu=full["nu"], uinf=full["uinf"],
+                       precision=pa.FP32C)
+    S.solve()
+    assert S.variant()["block"] == 256
+    for a, b in ((n0, n1), (n1, full["n"])):
+        S.appendParticles(0, full["pos"][a:b], full["vort"][a:b], full["radius"][a:b],
+                          full["volume"][a:b])
+        v = S.variant()
+        print(f"  after append to {b}: {v}", flush=True)
+        assert v["block"] == (256 if b <= 524288 else 512)
+    x = S.getStates()
+    n = full["n"]
+    assert np.array_equal(x[:3 * n], full["pos"].ravel())
+    assert np.array_equal(x[3 * n:], full["vort"].ravel())
+    S.solve()
+    r = S.getRates()
+    S.close()
+    rf, vf = _fp32c_rates(full)
+    assert vf["block"] == 512
+    assert np.array_equal(r, rf)
