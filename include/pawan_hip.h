@@ -201,6 +201,32 @@ int pw_get_variant(pw_ctx *ctx, pw_variant *out);
  * ms of the pair-interaction kernel alone, plus the pairs it evaluated */
 int pw_last_timing(const pw_ctx *ctx, double *total_ms, double *pair_ms, double *pairs);
 
+/* ------------------------------------------------------ no allocation
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the rates of one wake, as pw_solve + pw_get_rates give
+ * them for a context holding that wake (binary64, one-sided pass), with
+ * every intermediate (particle records, partial sums, work tables, counters)
+ * in a fixed array of this library in each device's memory
+ * (pw_noalloc_scratch_bytes(), under 32 MiB, held until the process ends).
+ *   pos, vort  N x 3 row-major, radius, volume  N: DEVICE arrays (inputs)
+ *   vel, dvort N x 3 row-major: DEVICE arrays (outputs)
+ *   uinf       host array of 3 (NULL: no freestream); stream: a hipStream_t
+ *              (NULL: the default stream) on the current device
+ * Asynchronous: returns once the kernels are queued on `stream`.  At most
+ * PW_NOALLOC_MAX_N particles (PW_EINVAL beyond, with a message).  A
+ * non-finite rate is not reported by the return code (no host
+ * synchronisation); it shows in the outputs.  Every call on a device shares
+ * that device's array, so calls that overlap on one device (on different
+ * streams or host threads) race; calls in order on one stream do not.  The
+ * first call on each device sets up (looks up the array's address there). */
+#define PW_NOALLOC_MAX_N 16384
+int pw_rates_noalloc(int kernel, double nu, const double *uinf, size_t n, const double *pos,
+                     const double *vort, const double *radius, const double *volume, double *vel,
+                     double *dvort, void *stream);
+/* bytes of the per-device scratch array of pw_rates_noalloc */
+size_t pw_noalloc_scratch_bytes(void);
+
 /* ------------------------------------------------------------------ batch
  * Many independent simulations stepped together: one persistent launch over
  * a table of (job, target tile, source chunk) work items per RHS, and the RK4

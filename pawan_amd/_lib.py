@@ -79,6 +79,9 @@ SIGNATURES = {
     "pw_diagnostics": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "pw_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "pw_get_variant": (C.c_int, [C.c_void_p, C.POINTER(PwVariant)]),
+    "pw_rates_noalloc": (C.c_int, [C.c_int, C.c_double, _dp, C.c_size_t, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pw_noalloc_scratch_bytes": (C.c_size_t, []),
     "pw_diagnose": (C.c_int, [C.c_void_p, C.POINTER(PwDiag)]),
     "pw_resolve": (C.c_int, [C.c_void_p, _dp]),
     "pw_write": (C.c_int, [C.c_void_p, C.c_void_p]),

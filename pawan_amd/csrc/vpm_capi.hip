@@ -486,6 +486,19 @@ int pw_get_variant(pw_ctx *ctx, pw_variant *out) {
     });
 }
 
+int pw_rates_noalloc(int kernel, double nu, const double *uinf, size_t n, const double *pos,
+                     const double *vort, const double *radius, const double *volume, double *vel,
+                     double *dvort, void *stream) {
+    static_assert(PW_NOALLOC_MAX_N == 16384, "header limit matches the scratch layout");
+    return guarded(nullptr, [&] {
+        if (pw::noalloc_max_n() != PW_NOALLOC_MAX_N) throw Error(PW_EINVAL, "scratch layout mismatch");
+        pw::rates_noalloc(kernel, nu, uinf, n, pos, vort, radius, volume, vel, dvort,
+                          (hipStream_t)stream);
+    });
+}
+
+size_t pw_noalloc_scratch_bytes(void) { return pw::noalloc_scratch_bytes(); }
+
 int pw_last_timing(const pw_ctx *ctx, double *total_ms, double *pair_ms, double *pairs) {
     if (!ctx || !ctx->eng) return PW_EINVAL;
     if (total_ms) *total_ms = ctx->eng->last_total_ms;

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/pawan_hip.h"
 
@@ -923,6 +924,188 @@ bool Engine::take_flag(Job *j) {
     PW_HIP(hipMemsetAsync(j->flag.p, 0, sizeof(int), stream_));
     PW_HIP(hipStreamSynchronize(stream_));
     return f != 0;
+}
+
+
+// ------------------------------------------------ rates without allocation
+// One wake's rates (a binary64 solve) for callers that cannot allocate device
+// memory: every intermediate lives in `scratch`, a fixed array of this
+// library in each device's memory, laid out for at most NOALLOC_MAX_N
+// particles.  Calls on one device share it, so calls that overlap on a
+// device race; each call rewrites every region it reads before reading it.
+namespace {
+
+constexpr size_t NOALLOC_MAX_N = 16384;
+constexpr int NOALLOC_TB = 256;  // one-sided binary64 pass, 256 x 1, unroll 2
+constexpr size_t NOALLOC_MAX_CHUNKS = 16;
+constexpr size_t NOALLOC_MAX_ITEMS = (NOALLOC_MAX_N / NOALLOC_TB) * NOALLOC_MAX_CHUNKS;
+
+constexpr size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+// byte offsets of the regions
+constexpr size_t NA_JOB = 0;
+constexpr size_t NA_ITEMS = al256(NA_JOB + sizeof(JobDev));
+constexpr size_t NA_CITEMS = al256(NA_ITEMS + NOALLOC_MAX_ITEMS * sizeof(Item));
+constexpr size_t NA_INTS = al256(NA_CITEMS + (NOALLOC_MAX_N / 256) * sizeof(CItem));
+// ints: [0] pair items [1] (unused) [2] combine items [3] pad | [4] [5] queue
+// counters | [8] non-finite flag
+constexpr size_t NA_REC = al256(NA_INTS + 16 * sizeof(int));
+constexpr size_t NA_XS = al256(NA_REC + NOALLOC_MAX_N * sizeof(Rec64));
+constexpr size_t NA_VOL = al256(NA_XS + 6 * NOALLOC_MAX_N * sizeof(double));
+constexpr size_t NA_RATE = al256(NA_VOL + NOALLOC_MAX_N * sizeof(double));
+constexpr size_t NA_PART = al256(NA_RATE + 6 * NOALLOC_MAX_N * sizeof(double));
+constexpr size_t NOALLOC_SCRATCH_BYTES =
+    al256(NA_PART + NOALLOC_MAX_CHUNKS * NCOMP * NOALLOC_MAX_N * sizeof(double));
+static_assert(NOALLOC_SCRATCH_BYTES < (32u << 20), "scratch stays under 32 MiB");
+
+}  // namespace
+
+size_t noalloc_max_n() { return NOALLOC_MAX_N; }
+size_t noalloc_scratch_bytes() { return NOALLOC_SCRATCH_BYTES; }
+
+// The scratch array.  In host code its name is a stand-in, not a device
+// address: host code uses only the per-device address from
+// hipGetSymbolAddress below.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+// Work tables, job descriptor, zeroed counters and flag, and the records of
+// the caller's particles (N x 3 positions / vorticities, N radii / volumes,
+// device arrays) written into the scratch regions: the plan build and record
+// fill of the context path, on the device, so that the call needs no copy
+// from host memory.
+__global__ __launch_bounds__(256) void noalloc_prepare_kernel(
+    JobDev J, int ntile, int nchunk, Item *__restrict__ items, CItem *__restrict__ citems,
+    int *__restrict__ ints, JobDev *__restrict__ jd, const double *__restrict__ pos,
+    const double *__restrict__ vort, const double *__restrict__ radius,
+    const double *__restrict__ volume) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int nitems = ntile * nchunk, nci = (J.n + 255) / 256;
+    if (k == 0) {
+        *jd = J;
+        ints[0] = nitems;
+        ints[1] = 0;
+        ints[2] = nci;
+        ints[3] = 0;
+        ints[4] = 0;  // the pair kernel's queue
+        ints[5] = 0;
+        ints[8] = 0;  // non-finite flag
+    }
+    if (k < nitems) items[k] = Item{0, k % ntile, k / ntile, J.chunk_len};
+    if (k < nci) citems[k] = CItem{0, k};
+    if (k < J.n) {
+        const size_t np = (size_t)J.npad;
+        Rec64 r;
+        r.x = pos[3 * (size_t)k];
+        r.y = pos[3 * (size_t)k + 1];
+        r.z = pos[3 * (size_t)k + 2];
+        r.h = 0.5 * radius[k] * radius[k];
+        r.ax = vort[3 * (size_t)k];
+        r.ay = vort[3 * (size_t)k + 1];
+        r.az = vort[3 * (size_t)k + 2];
+        r.v = volume[k];
+        ((Rec64 *)J.rec)[k] = r;
+        J.xs[3 * np + k] = r.ax;
+        J.xs[4 * np + k] = r.ay;
+        J.xs[5 * np + k] = r.az;
+        ((double *)J.vol)[k] = r.v;
+    }
+}
+
+namespace {
+struct NoallocDevice {
+    bool ready = false;
+    unsigned char *ptr = nullptr;  // device address of scratch on this device
+    int grid = 0;                  // persistent grid of the pair kernel
+};
+std::mutex g_noalloc_mu;
+NoallocDevice g_noalloc[64];
+}  // namespace
+
+void rates_noalloc(int kind, double nu, const double uinf[3], size_t n, const double *pos,
+                   const double *vort, const double *radius, const double *volume, double *vel,
+                   double *dvort, hipStream_t stream) {
+    if (kind != KIND_HIGHORDER && kind != KIND_GAUSSIAN) throw Error(PW_EINVAL, "unknown kernel");
+    if (n == 0 || !pos || !vort || !radius || !volume || !vel || !dvort)
+        throw Error(PW_EINVAL, "empty system or null array");
+    if (n > NOALLOC_MAX_N)
+        throw Error(PW_EINVAL, "pw_rates_noalloc takes at most " + std::to_string(NOALLOC_MAX_N) +
+                                   " particles (its fixed device scratch is laid out for that)");
+    const double u[3] = {uinf ? uinf[0] : 0.0, uinf ? uinf[1] : 0.0, uinf ? uinf[2] : 0.0};
+    if (!std::isfinite(nu) || !std::isfinite(u[0]) || !std::isfinite(u[1]) || !std::isfinite(u[2]))
+        throw Error(PW_EINVAL, "non-finite parameter");
+    PairFn fn = kind == KIND_HIGHORDER ? pair64_kernel<KIND_HIGHORDER, NOALLOC_TB, 1, 2>
+                                       : pair64_kernel<KIND_GAUSSIAN, NOALLOC_TB, 1, 2>;
+    int dev = 0;
+    PW_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw Error(PW_EINVAL, "device ordinal beyond 63");
+    NoallocDevice *D = &g_noalloc[dev];
+    {
+        // set-up once per device, with that device current: the array's
+        // address on it, and the pair kernel's resident grid
+        std::lock_guard<std::mutex> g(g_noalloc_mu);
+        if (!D->ready) {
+            void *p = nullptr;
+            PW_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            int cus = 0, b64 = 0, b64g = 0;
+            PW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            PW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &b64, (const void *)pair64_kernel<KIND_HIGHORDER, NOALLOC_TB, 1, 2>, NOALLOC_TB, 0));
+            PW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &b64g, (const void *)pair64_kernel<KIND_GAUSSIAN, NOALLOC_TB, 1, 2>, NOALLOC_TB, 0));
+            D->ptr = (unsigned char *)p;
+            D->grid = std::max(1, std::min(b64, b64g)) * cus;
+            D->ready = true;
+        }
+    }
+    unsigned char *base = D->ptr;
+    // the job as the context path would describe it, with every array in the
+    // scratch regions; cores are taken as non-uniform (the radii are on the
+    // device), so sigma is formed per pair
+    JobDev J;
+    std::memset(&J, 0, sizeof J);
+    const size_t npad = round_up(n, 256), L = chunk_for(n, NOALLOC_TB);
+    J.rec = base + NA_REC;
+    J.xs = (double *)(base + NA_XS);
+    J.x0 = J.xs;
+    J.rate = (double *)(base + NA_RATE);
+    J.part = (double *)(base + NA_PART);
+    J.vol = (const double *)(base + NA_VOL);
+    J.flag = (int *)(base + NA_INTS) + 8;
+    J.n = (int)n;
+    J.npad = (int)npad;
+    J.chunk_len = (int)L;
+    J.nchunk = (int)((n + L - 1) / L);
+    J.prec = PREC_FP64;
+    J.uniform = 0;
+    J.inv_s2 = 0.0;
+    J.rec_scale = 1.0;
+    J.nu = nu;
+    for (int c = 0; c < 3; ++c) J.uinf[c] = u[c];
+    const PairScale ps = pair_scale(kind, PREC_FP64);
+    J.su = J.sq = J.sf = (double)(1.0L / (long double)ps.q);
+    J.sd = (double)(2.0L * nu / (long double)ps.z);
+    const int ntile = (int)((n + NOALLOC_TB - 1) / NOALLOC_TB);
+    if ((size_t)J.nchunk > NOALLOC_MAX_CHUNKS || (size_t)ntile * J.nchunk > NOALLOC_MAX_ITEMS)
+        throw Error(PW_EINVAL, "work plan exceeds the scratch tables");
+    JobDev *jd = (JobDev *)(base + NA_JOB);
+    Item *items = (Item *)(base + NA_ITEMS);
+    CItem *citems = (CItem *)(base + NA_CITEMS);
+    int *ints = (int *)(base + NA_INTS);
+    const int nprep = (int)std::max<size_t>(n, (size_t)ntile * J.nchunk);
+    hipLaunchKernelGGL(noalloc_prepare_kernel, dim3((nprep + 255) / 256), dim3(256), 0, stream, J,
+                       ntile, J.nchunk, items, citems, ints, jd, pos, vort, radius, volume);
+    PW_HIP(hipGetLastError());
+    hipLaunchKernelGGL(fn, dim3(D->grid), dim3(NOALLOC_TB), 0, stream, (const JobDev *)jd,
+                       (const Item *)items, (const int *)ints, ints + 4);
+    PW_HIP(hipGetLastError());
+    hipLaunchKernelGGL(combine_kernel, dim3(4 * std::max(1, ntile)), dim3(256), 0, stream,
+                       (const JobDev *)jd, (const CItem *)citems, (const int *)(ints + 2), ints + 4,
+                       (int)MODE_SOLVE);
+    PW_HIP(hipGetLastError());
+    hipLaunchKernelGGL(aos_soa_kernel, dim3(grid_for(n)), dim3(256), 0, stream, vel, J.rate, (int)n,
+                       (int)npad, 1);
+    hipLaunchKernelGGL(aos_soa_kernel, dim3(grid_for(n)), dim3(256), 0, stream, dvort,
+                       J.rate + 3 * npad, (int)n, (int)npad, 1);
+    PW_HIP(hipGetLastError());
 }
 
 }  // namespace pw

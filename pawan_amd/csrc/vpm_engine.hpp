@@ -116,6 +116,16 @@ struct Variant {
 bool variant_exists(int kind, int tb, int tpt, int unr64, int unr32, int sym32 = 1,
                     int sym64 = 0, int tpt32 = 0);
 
+// One wake's binary64 rates without allocating device memory (every
+// intermediate in a fixed per-device scratch array of this library; all
+// pointers are device arrays; asynchronous on `stream`).  At most
+// noalloc_max_n() particles.  Calls that overlap on one device race.
+void rates_noalloc(int kind, double nu, const double uinf[3], size_t n, const double *pos,
+                   const double *vort, const double *radius, const double *volume, double *vel,
+                   double *dvort, hipStream_t stream);
+size_t noalloc_max_n();
+size_t noalloc_scratch_bytes();
+
 class Engine {
   public:
     Engine(int device, int kind, const Variant &v);
