@@ -13,10 +13,18 @@ Prints one JSON line:
   roofline    the binary64 pair kernel's algorithmic FLOP rate (95 FLOP per
               ordered pair) over its mean launch time from HIP events on its
               stream, against the FP64 vector peak
+  drift       total vorticity, linear impulse and kinetic energy before the
+              warm-up steps and after the timed steps, per path
   cpu_baseline  the OpenMP CPU oracle (pawan's __parallel loop restated, with
               its per-pair heap temporaries) on a stratified sample of source
-              rows of the same workload, on this host's cores
-  serving     a short continuous-batching run: p50/p99 step and job latency
+              rows of the same workload, on this host's cores; and the
+              oracle's rates of a stratified sample of targets (against all
+              262,144 sources), which each path's rates are checked against:
+              fp32c.rel_rms_err_vs_fp64_oracle (per term) and
+              max_rel_err_vs_fp64_oracle (binary64, per particle)
+  serving     the serving stream (N log-uniform in [2,048, 262,144], 50-500
+              steps, binary64 / binary32 50/50, 200 GB budget) for a window of
+              Poisson arrivals: p50/p99 step and job latency, unfinished jobs
 """
 from __future__ import annotations
 
@@ -50,11 +58,12 @@ def log(*a):
 # The synthetic rotor wake (tip vortex of strength 1 in cores of 0.02) is an
 # unstable particle set: vorticity grows with an e-folding time of about 1 ms
 # of simulated time whatever the step (the CPU oracle shows the same: NaN by
-# t = 0.08 at dt 1e-3, 1e17 by t = 0.03 at dt 2.5e-4).  Short timed runs use
-# STEP_DT; runs of hundreds of steps (growth, serving) use LONG_DT so that the
-# simulated time stays near 10 ms and every state stays finite.  The step
-# costs the same for any dt.
-STEP_DT = 1e-3
+# t = 0.08 at dt 1e-3, 1e17 by t = 0.03 at dt 2.5e-4).  Its runs use
+# STEP_DT = 1e-5, so that over a timed run of tens of steps (and the growth
+# and serving runs of hundreds) the simulated time stays well under 1 ms, the
+# state stays finite and the drift of the invariants means something.  A
+# step costs the same for any dt.
+STEP_DT = 1e-5
 LONG_DT = 1e-5
 
 # The variants the autotuner picks for the N = 262,144 rotor wake on MI355X
@@ -76,12 +85,14 @@ def run_path(pa, w, precision, steps, warmup, variant=None):
                        unroll=v.get("unroll", 0), scheme=v.get("scheme", 0))
     dt = STEP_DT
     S.solve()  # builds the device job
+    d0 = S.diagnose()  # before the warm-up steps (outside the timed region)
     for _ in range(warmup):
         S.rk4(dt, 1)
     t0 = time.perf_counter()
     S.rk4(dt, steps)
     wall = time.perf_counter() - t0
     tm = S.last_timing()
+    d1 = S.diagnose()
     n = w["n"]
     pairs = 4.0 * steps * n * (n - 1)
     launches = 4 * steps
@@ -90,6 +101,7 @@ def run_path(pa, w, precision, steps, warmup, variant=None):
         "pair_kernel_ms_mean": tm["pair_ms"] / launches,
         "pairs_per_launch": n * (n - 1.0),
         "pairs_per_s": pairs / wall,
+        "drift": dict(_drift(d0, d1), steps=warmup + steps, dt=dt),
     }
     return S, out
 
@@ -131,7 +143,66 @@ def split_error_for(pa, w, nu, uinf=(0.0, 0.0, 0.0)):
     return e
 
 
-def cpu_baseline(w, budget_s=15.0):
+def gpu_terms(S, w):
+    """The path's rates at the workload's initial state, per term: velocity,
+    vorticity rate, stretching (nu = 0) and diffusion (their difference)."""
+    n = w["n"]
+    S.setStates(np.concatenate([w["pos"].ravel(), w["vort"].ravel()]))
+    S.solve()
+    r = S.getRates()
+    S.setNu(0.0)
+    S.solve()
+    r0 = S.getRates()
+    S.setNu(w["nu"])
+    v, d = r[:3 * n].reshape(n, 3), r[3 * n:].reshape(n, 3)
+    st = r0[3 * n:].reshape(n, 3)
+    return v, d, st, d - st
+
+
+def sample_targets(n, k, seed=0):
+    """About k target indices: one per equal stratum (seeded), plus both ends
+    and the indices either side of every 16,384th (where source chunks split)."""
+    rng = np.random.default_rng(seed)
+    b = np.linspace(0, n, k + 1).astype(np.int64)
+    idx = [int(rng.integers(lo, hi)) for lo, hi in zip(b[:-1], b[1:]) if hi > lo]
+    idx += [0, n - 1] + [m + e for m in range(16384, n, 16384) for e in (-1, 0)]
+    return np.unique(np.clip(idx, 0, n - 1))
+
+
+def oracle_check(orc, w, gpu, targets):
+    """Rates of the sampled targets from the oracle (the reference's binary64
+    pair arithmetic, every source) against each GPU path's: binary64 max
+    per-particle relative error (floor 1e-3 RMS), binary32 relative RMS per term."""
+    t0 = time.perf_counter()
+    rv, rs, rf = orc.rates_sample(w["pos"], w["vort"], w["radius"], w["volume"], w["nu"],
+                                  targets, uinf=np.asarray(w.get("uinf", (0.0, 0.0, 0.0))))
+    secs = time.perf_counter() - t0
+    rd = rs + rf
+
+    def per_particle(a, b):
+        nb = np.linalg.norm(b, axis=1)
+        fl = 1e-3 * np.sqrt(np.mean(nb * nb))
+        return float(np.max(np.linalg.norm(a - b, axis=1) / np.maximum(nb, fl)))
+
+    def rr(a, b):
+        return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+    out = {"targets": int(len(targets)), "oracle_seconds": secs}
+    for name, (v, d, st, df) in gpu.items():
+        v, d, st, df = v[targets], d[targets], st[targets], df[targets]
+        if name.startswith("fp64"):
+            out[name] = {"max_rel_err_velocity": per_particle(v, rv),
+                         "max_rel_err_vorticity_rate": per_particle(d, rd)}
+        else:
+            out[name] = {"velocity": rr(v, rv), "stretching": rr(st, rs), "diffusion": rr(df, rf),
+                         "vorticity_rate": rr(d, rd)}
+    return out
+
+
+def cpu_baseline(w, budget_s=15.0, gpu=None, n_sample=512):
+    """The CPU leg: the reference-faithful OpenMP loop timed on a sample of
+    source rows, and (gpu given) the oracle's rates of a sample of targets,
+    which every GPU path's rates at the same state are checked against."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc  # CPU baseline leg only
 
@@ -148,27 +219,36 @@ def cpu_baseline(w, budget_s=15.0):
     pairs, secs = orc.omp_rows(w["pos"], w["vort"], w["radius"], w["volume"], w["nu"],
                                stride // 2, stride, nrows, faithful=True)
     ordered = 2.0 * pairs  # each unordered pair updates both particles
-    return {
+    out = {
         "value": ordered / secs, "unit": "pair-interactions/s", "cores": orc.max_threads(),
         "kind": "port",
         "sample": (f"{nrows} of {n} source rows (stride {stride}) of one RHS of the same "
                    f"rotor wake, OpenMP loop of parallel.cpp:13-48 with per-pair heap "
                    f"temporaries; {pairs:.3e} unordered pairs in {secs:.1f} s"),
     }
+    if gpu:
+        out["oracle_check"] = oracle_check(orc, w, gpu, sample_targets(n, n_sample))
+    return out
 
 
-def serving_run(pa, gen, duration_s=20.0, rate_hz=4.0, seed=1, budget=200 << 30, n_lo=2048,
-                n_hi=32768, s_lo=50, s_hi=200, drain_factor=3.0, check_failed=False):
-    """Poisson arrivals for duration_s; ring or rotor-wake jobs with N
-    log-uniform in [n_lo, n_hi], steps uniform in [s_lo, s_hi], precision
-    50/50; runs until all arrived jobs finish or drain_factor x duration_s
-    passes (jobs still running then are not counted as finished).
+def serving_run(pa, gen, duration_s=120.0, rate_hz=0.1, seed=1, budget=200 * 10 ** 9, n_lo=2048,
+                n_hi=262144, s_lo=50, s_hi=500, drain_factor=1.5, check_failed=False):
+    """The serving stream (generators.serving_stream): Poisson arrivals at
+    rate_hz for duration_s; ring or rotor-wake jobs with N log-uniform in
+    [n_lo, n_hi], steps uniform in [s_lo, s_hi], precision 50/50, admitted
+    against an HBM budget of `budget` bytes.  Runs until every arrived job
+    has finished or drain_factor x duration_s has passed; jobs still running
+    or queued then are reported as unfinished (their latency is not in the
+    job percentiles).
     check_failed: rerun every failed binary32 job alone in binary64 for the
     steps it completed and report the largest |state| it reaches (a job
     that diverges there too diverges under the method itself)."""
-    u = gen.uniform(seed, 4096)
+    count = int(rate_hz * duration_s * 3) + 16
+    stream = [j for j in gen.serving_stream(count, seed=seed, rate_hz=rate_hz, n_lo=n_lo,
+                                            n_hi=n_hi, s_lo=s_lo, s_hi=s_hi, rotor_dt=LONG_DT)
+              if j["arrival_ms"] < duration_s * 1000.0]
     srv = pa.Server(budget)
-    t_next, k, ids, meta, specs = 0.0, 0, [], {}, {}
+    k, ids, meta = 0, [], {}
     t_end = duration_s * 1000.0
     t_log = 0.0
     while True:
@@ -178,44 +258,34 @@ def serving_run(pa, gen, duration_s=20.0, rate_hz=4.0, seed=1, budget=200 << 30,
                 f"running {st['running']} queued {st['queued']} finished {st['finished']}")
             t_log += 20000.0
         now = srv.now_ms()
-        while now >= t_next and t_next < t_end:
-            a = u[(6 * k) % 4090: (6 * k) % 4090 + 6]
-            n = int(n_lo * (n_hi / n_lo) ** a[0])
-            steps = int(s_lo + (s_hi - s_lo) * a[1])
-            prec = pa.FP64 if a[2] < 0.5 else pa.FP32C
-            if a[3] < 0.5:
-                n_cs = gen.ring_cs_for(n, 1.0, 0.1)
-                w = gen.vortex_ring(max(8, n // n_cs), n_cs, R=1.0, core=0.1, seed=seed + k)
-                ids.append(srv.submit(w, 1e-3, 0.005, steps, prec, arrival_ms=t_next))
-                specs[ids[-1]] = (w, 1e-3, 0.005, (0.0, 0.0, 0.0))
-            else:
-                w = gen.rotor_wake(max(1, n // 256))
-                ids.append(srv.submit(w, 1e-4, LONG_DT, steps, prec, uinf=(0, 0, -0.05),
-                                      arrival_ms=t_next))
-                specs[ids[-1]] = (w, 1e-4, LONG_DT, (0.0, 0.0, -0.05))
-            meta[ids[-1]] = ("ring" if a[3] < 0.5 else "rotor", w["pos"].shape[0],
-                             "fp64" if prec == pa.FP64 else "fp32c", steps)
+        while k < len(stream) and now >= stream[k]["arrival_ms"]:
+            j = stream[k]
+            ids.append(srv.submit(j["w"], j["nu"], j["dt"], j["steps"], j["precision"],
+                                  uinf=j["uinf"], arrival_ms=j["arrival_ms"]))
+            meta[ids[-1]] = (j["kind"], j["n"], "fp64" if j["precision"] == 0 else "fp32c",
+                             j["steps"], k)
             k += 1
-            t_next += -math.log(1.0 - a[4]) / rate_hz * 1000.0
         running = srv.step()
         st = srv.stats()
-        if t_next >= t_end and running == 0 and st["queued"] == 0:
+        if k == len(stream) and running == 0 and st["queued"] == 0:
             break
         if srv.now_ms() > drain_factor * t_end:
             break
-        if running == 0 and st["queued"] == 0:
-            time.sleep(max(0.0, min(t_next - srv.now_ms(), 50.0)) / 1000.0)
+        if running == 0 and st["queued"] == 0 and k < len(stream):
+            time.sleep(max(0.0, min(stream[k]["arrival_ms"] - srv.now_ms(), 50.0)) / 1000.0)
     st = srv.stats()
-    failed = []
+    failed, unfinished = [], []
     for i in ids:
         state, done = srv.state(i)
+        if state in (srv.QUEUED, srv.RUNNING):
+            unfinished.append(list(meta[i][:4]) + [done])
         if state == srv.FAILED:
-            failed.append(list(meta[i]) + [done])
+            failed.append(list(meta[i][:4]) + [done])
             if check_failed and meta[i][2] == "fp32c":
-                w, nu, dt, uinf = specs[i]
-                S = pa.Interaction(pa.Wake.from_dict(w), nu=nu, uinf=uinf)
+                j = stream[meta[i][4]]
+                S = pa.Interaction(pa.Wake.from_dict(j["w"]), nu=j["nu"], uinf=j["uinf"])
                 try:
-                    S.rk4(dt, done)
+                    S.rk4(j["dt"], done)
                     fin = True
                 except pa.PawanError as e:
                     if e.code != 4:
@@ -225,22 +295,23 @@ def serving_run(pa, gen, duration_s=20.0, rate_hz=4.0, seed=1, budget=200 << 30,
                 S.close()
                 ok = np.isfinite(x)
                 failed[-1].append({"fp64_finite": fin, "fp64_max_abs_state": float(np.abs(x[ok]).max()),
-                                   "max_abs_state_t0": float(np.abs(w["pos"]).max())})
-                log(f"  failed {meta[i]} at step {done}: alone in binary64 {failed[-1][-1]}")
+                                   "max_abs_state_t0": float(np.abs(j["w"]["pos"]).max())})
+                log(f"  failed {meta[i][:4]} at step {done}: alone in binary64 {failed[-1][-1]}")
     srv.close()
+    pairs_per_s = st["pairs"] / max(st["busy_ms"], 1e-9) * 1000.0
     return {
         "failed_jobs": failed,  # [kind, n, precision, steps asked, steps done(, binary64 check)]
-        "arrival_rate_hz": rate_hz, "duration_s": duration_s, "jobs": st["submitted"],
-        "finished": st["finished"], "failed": st["failed"],
+        "unfinished_jobs": unfinished,  # [kind, n, precision, steps asked, steps done]
+        "arrival_rate_hz": rate_hz, "duration_s": duration_s, "stop_after_s": drain_factor * duration_s,
+        "jobs": st["submitted"], "finished": st["finished"], "failed": st["failed"],
         "step_p50_ms": st["step_p50_ms"], "step_p99_ms": st["step_p99_ms"],
         "job_p50_ms": st["job_p50_ms"], "job_p99_ms": st["job_p99_ms"],
-        "pairs_per_s": st["pairs"] / max(st["busy_ms"], 1e-9) * 1000.0,
+        "pairs_per_s": pairs_per_s,
         # the served mix at the binary64 method's 95 algorithmic FLOP per ordered
         # pair against the FP64 vector peak (binary32 jobs count at that rate too)
-        "frac_fp64_peak_95flop": (st["pairs"] / max(st["busy_ms"], 1e-9) * 1000.0 *
-                                  FLOP_PER_PAIR / FP64_VECTOR_PEAK),
+        "frac_fp64_peak_95flop": pairs_per_s * FLOP_PER_PAIR / FP64_VECTOR_PEAK,
         "steps": st["steps"], "n_range": [n_lo, n_hi], "steps_range": [s_lo, s_hi],
-        "still_running": st["running"] + st["queued"],
+        "hbm_budget_gb": budget / 1e9, "still_running": st["running"] + st["queued"],
     }
 
 
@@ -266,9 +337,13 @@ def copy_calibration(mib):
 
 # ------------------------------------------------------------------ workloads
 def _drift(d0, d1):
+    """Change of total vorticity and linear impulse (interaction.cpp:240-259):
+    |change| and |change| / |initial|; kinetic energy (:351-366): relative change."""
     out = {}
     for k in ("total_vorticity", "linear_impulse"):
-        out[k] = float(np.linalg.norm(np.asarray(d1[k]) - np.asarray(d0[k])))
+        a, b = np.asarray(d0[k]), np.asarray(d1[k])
+        out[k] = float(np.linalg.norm(b - a))
+        out[k + "_rel"] = float(np.linalg.norm(b - a) / max(np.linalg.norm(a), 1e-300))
     out["kinetic_energy_rel"] = float((d1["kinetic_energy"] - d0["kinetic_energy"]) /
                                       d0["kinetic_energy"])
     return out
@@ -539,9 +614,9 @@ def bench_serve(pa, gen, args):
     """Serving stream: Poisson arrivals at --rate per second for
     --serve-seconds; each job a ring or rotor wake with N log-uniform in
     [2,048, 262,144], steps uniform in [50, 500], FP64 or FP32c 50/50, 200 GB
-    HBM budget."""
-    r = serving_run(pa, gen, duration_s=args.serve_seconds, rate_hz=args.rate, n_lo=2048,
-                    n_hi=262144, s_lo=50, s_hi=500, drain_factor=args.drain, check_failed=True)
+    HBM budget; stops after --drain x --serve-seconds."""
+    r = serving_run(pa, gen, duration_s=args.serve_seconds, rate_hz=args.rate,
+                    drain_factor=args.drain, check_failed=True)
     return {"metric": "serving: p50/p99 step and job latency, aggregate pair-interactions/s",
             "value": r["pairs_per_s"], "unit": "pair-interactions/s", "n_gpus": 1, "dtype": "mixed",
             "config": {"workload": "serving_stream", "n_range": [2048, 262144],
@@ -558,7 +633,8 @@ def main():
     ap.add_argument("--no-fp32c", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=150.0,
+                    help="serving: arrival window in seconds")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (twice, fresh buffers): the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -569,8 +645,8 @@ def main():
                              "frames"])
     ap.add_argument("--shed-steps", type=int, default=1024)
     ap.add_argument("--jobs", type=int, default=256)
-    ap.add_argument("--rate", type=float, default=2.0, help="serving arrivals per second")
-    ap.add_argument("--drain", type=float, default=3.0, help="serving: stop after drain x duration")
+    ap.add_argument("--rate", type=float, default=0.2, help="serving arrivals per second")
+    ap.add_argument("--drain", type=float, default=1.6, help="serving: stop after drain x duration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -638,19 +714,28 @@ def main():
                      "kernel": "pair64_kernel", "flop_per_pair": FLOP_PER_PAIR,
                      "launch_ms": r64["pair_kernel_ms_mean"]},
         "variants": tuned, "autotuned": not args.no_tune,
+        # total vorticity, linear impulse and kinetic energy before the warm-up
+        # steps and after the timed ones (outside the timed region)
+        "drift": r64["drift"],
     }
+    # each path's rates at the initial state, checked against the oracle in
+    # the CPU leg
+    gpu = {"fp64": gpu_terms(S64, w)}
+
+    def rel_pp(a, b):
+        a, b = a.reshape(-1, 3), b.reshape(-1, 3)
+        nb = np.linalg.norm(b, axis=1)
+        return np.linalg.norm(a - b, axis=1) / np.maximum(nb, 1e-3 * np.sqrt(np.mean(nb * nb)))
+
+    def rr(a, b):
+        return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
     if "fp64_symmetric" in tuned:
         log("fp64 symmetric path")
         Ssym, rsym = run_path(pa, w, pa.FP64, args.steps, args.warmup, tuned["fp64_symmetric"])
-        x0 = np.concatenate([w["pos"].ravel(), w["vort"].ravel()])
-        agree = {}
-        for S in (S64, Ssym):
-            S.setStates(x0)
-            S.solve()
-            agree[S] = S.getRates()
-        a, b = agree[Ssym].reshape(-1, 3), agree[S64].reshape(-1, 3)
-        nb = np.linalg.norm(b, axis=1)
-        rel = np.linalg.norm(a - b, axis=1) / np.maximum(nb, 1e-3 * np.sqrt(np.mean(nb * nb)))
+        gpu["fp64_symmetric"] = gpu_terms(Ssym, w)
+        rel = np.maximum(rel_pp(gpu["fp64_symmetric"][0], gpu["fp64"][0]),
+                         rel_pp(gpu["fp64_symmetric"][1], gpu["fp64"][1]))
         line["fp64_symmetric"] = {
             "value": rsym["pairs_per_s"], "ms_per_step": rsym["wall_s"] * 1000.0 / args.steps,
             "pair_kernel_ms": rsym["pair_kernel_ms_mean"],
@@ -659,12 +744,16 @@ def main():
             "frac_fp64_peak": FLOP_PER_PAIR_SYM * rsym["pairs_per_launch"] /
             (rsym["pair_kernel_ms_mean"] * 1e-3) / FP64_VECTOR_PEAK,
             "max_rel_rate_diff_vs_plain": float(rel.max()),
+            "drift": rsym["drift"],
         }
         Ssym.close()
     if not args.no_fp32c:
         log("fp32c path")
         S32, r32 = run_path(pa, w, pa.FP32C, args.steps, args.warmup, tuned.get("fp32c"))
-        err = rates_split_error(S64, S32, w)
+        gpu["fp32c"] = g32 = gpu_terms(S32, w)
+        g64 = gpu["fp64"]
+        err = {"velocity": rr(g32[0], g64[0]), "stretching": rr(g32[2], g64[2]),
+               "diffusion": rr(g32[3], g64[3]), "vorticity_rate": rr(g32[1], g64[1])}
         sym = tuned.get("fp32c", {}).get("scheme", pa.SCHEME_DEFAULT) != pa.SCHEME_ONESIDED
         flop32 = FLOP_PER_PAIR_SYM if sym else FLOP_PER_PAIR
         line["fp32c"] = {
@@ -679,15 +768,22 @@ def main():
             "frac_fp32_peak": flop32 * r32["pairs_per_launch"] /
             (r32["pair_kernel_ms_mean"] * 1e-3) / FP32_VECTOR_PEAK,
             "rel_rms_err_vs_fp64_gpu": err,
+            "drift": r32["drift"],
         }
         S32.close()
     S64.close()
     if not args.no_serving:
         log("serving run")
-        line["serving"] = serving_run(pa, gen, duration_s=args.serve_seconds)
+        line["serving"] = serving_run(pa, gen, duration_s=args.serve_seconds, rate_hz=args.rate,
+                                      drain_factor=args.drain)
     if not args.no_cpu:
         log("cpu baseline")
-        line["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
+        line["cpu_baseline"] = cb = cpu_baseline(w, args.cpu_budget, gpu=gpu)
+        chk = cb.get("oracle_check", {})
+        if "fp32c" in chk:
+            line["fp32c"]["rel_rms_err_vs_fp64_oracle"] = chk["fp32c"]
+        if "fp64" in chk:
+            line["max_rel_err_vs_fp64_oracle"] = chk["fp64"]
     print(json.dumps(line), flush=True)
 
 
