@@ -272,19 +272,33 @@ size_t pw_chunk_for(size_t n, int block);
 
 /* ---------------------------------------------------------------- serving
  * A continuous-batching request loop: jobs are queued with an arrival time,
- * admitted while their HBM footprint fits the budget, stepped together with
- * every other resident job, rebatched every step as jobs join and finish, and
- * retired with their final state kept until fetched. */
+ * admitted while their HBM footprint fits the budget, advanced together with
+ * every other resident job, rebatched every pass as jobs join and finish, and
+ * retired with their final state kept until fetched.
+ *
+ * A pass advances every resident job by 1, 2 or 4 RK4 stages (never past
+ * the end of its current step): a job whose stage costs at most a quarter
+ * (half) of the largest resident job's stage, or of a floor of 2e9 pair
+ * interactions, runs 4 (2) stages, the others 1, so that small jobs take
+ * whole steps while a large job takes one stage per pass, and no pass holds a
+ * whole step of the largest job.  Each stage is the same computation
+ * whichever pass it falls in, so a job's result does not depend on the mix.
+ * Jobs that arrive during a pass are uploaded beside it (a second stream)
+ * and join the next one; the non-finite flags of all jobs come back in one
+ * copy per pass. */
 typedef struct pw_server pw_server;
 
 typedef struct pw_server_stats {
     int64_t submitted, admitted, finished, failed, cancelled, queued, running;
-    int64_t steps;             /* batched steps executed                      */
+    int64_t steps;             /* passes executed                              */
     double pairs;              /* ordered pair-interactions evaluated         */
-    double busy_ms;            /* device time of the batched steps            */
-    double step_p50_ms, step_p99_ms; /* batched step latency                  */
+    double busy_ms;            /* device time of the passes                   */
+    double step_p50_ms, step_p99_ms; /* pass latency (admit, run, retire)     */
     double job_p50_ms, job_p99_ms;   /* arrival -> final state ready          */
     size_t hbm_in_use, hbm_budget;   /* bytes                                 */
+    double job_step_p50_ms, job_step_p99_ms; /* per finished job: admission ->
+                                       retirement / RK4 steps, the wall time
+                                       per step that job saw                  */
 } pw_server_stats;
 
 int pw_server_create(pw_server **out, int device, int kernel, size_t hbm_budget_bytes,
@@ -298,9 +312,10 @@ double pw_server_now_ms(const pw_server *s);
 int pw_server_submit(pw_server *s, const pw_job_spec *spec, size_t nsteps, double arrival_ms,
                      int64_t *id);
 int pw_server_cancel(pw_server *s, int64_t id);
-/* one pass of the request loop: admit queued jobs that fit the budget, run
- * one batched RK4 step over every resident job, retire finished and failed
- * jobs.  *running receives the number of resident jobs after the pass. */
+/* one pass of the request loop: queue a pass over every resident job (1, 2
+ * or 4 RK4 stages each), admit queued jobs that fit the budget while it
+ * runs, then retire finished and failed jobs.  *running receives the number
+ * of resident jobs after the pass. */
 int pw_server_step(pw_server *s, int64_t *running);
 /* 0 queued, 1 running, 2 done, 3 failed (non-finite), 4 cancelled, -1 unknown */
 int pw_server_job_state(pw_server *s, int64_t id, int64_t *steps_done);

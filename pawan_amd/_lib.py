@@ -55,7 +55,8 @@ class PwServerStats(C.Structure):
                 ("busy_ms", C.c_double), ("step_p50_ms", C.c_double),
                 ("step_p99_ms", C.c_double), ("job_p50_ms", C.c_double),
                 ("job_p99_ms", C.c_double), ("hbm_in_use", C.c_size_t),
-                ("hbm_budget", C.c_size_t)]
+                ("hbm_budget", C.c_size_t), ("job_step_p50_ms", C.c_double),
+                ("job_step_p99_ms", C.c_double)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/pawan_hip.h

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <map>
 #include <string>
 #include <vector>
@@ -597,8 +598,10 @@ int pw_batch_remove(pw_batch *b, int64_t id) {
 }
 
 static void batch_flags(pw_batch *b) {
-    for (auto &j : b->eng->jobs())
-        if (b->eng->take_flag(j.get())) b->status[j->id] = PW_ENONFINITE;
+    const std::vector<char> f = b->eng->take_flags();  // every job's, one copy
+    const auto &jobs = b->eng->jobs();
+    for (size_t k = 0; k < jobs.size(); ++k)
+        if (f[k]) b->status[jobs[k]->id] = PW_ENONFINITE;
 }
 
 int pw_batch_step(pw_batch *b, size_t nsteps, int use_graph) {
@@ -655,9 +658,28 @@ struct SJob {
     JobState state = JS_QUEUED;
     size_t n = 0, nsteps = 0, done = 0, bytes = 0;
     JobParams par;
-    double arrival = 0.0, finished = 0.0;
+    double arrival = 0.0, admitted = 0.0, finished = 0.0;
     std::vector<double> pos, vort, rad, vol;  // inputs until admitted, results when done
+    // while resident: the device job, the next RK4 stage (0-3) and the stages left
+    Job *job = nullptr;
+    int stage = 0;
+    size_t stages_left = 0;
 };
+
+// Stages a resident job runs in one pass: 4 if its stage costs at most a
+// quarter of the pass's reference cost (the largest resident stage, or
+// kPassFloor pair interactions if that is less), 2 if at most half, else 1;
+// never past the end of its current step or of its run.  A binary32 stage is
+// priced at half a binary64 one (its pair pass runs at about twice the rate).
+constexpr double kPassFloor = 2e9;
+double stage_cost(const SJob &j) {
+    return (double)j.n * (double)(j.n - 1) * (j.par.precision == PW_FP32C ? 0.5 : 1.0);
+}
+int stages_this_pass(const SJob &j, double ref) {
+    const double c = stage_cost(j);
+    const int k = 4 * c <= ref ? 4 : (2 * c <= ref ? 2 : 1);
+    return (int)std::min<size_t>({(size_t)k, (size_t)(4 - j.stage), j.stages_left});
+}
 
 double pct(std::vector<double> v, double q) {
     if (v.empty()) return 0.0;
@@ -674,10 +696,37 @@ struct pw_server {
     size_t budget = 0, in_use = 0;
     int64_t next_id = 1;
     std::map<int64_t, SJob> jobs;
-    std::deque<int64_t> queue;  // arrival order
+    std::deque<int64_t> queue;     // arrival order
+    std::vector<int64_t> resident;  // admission order
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
-    std::vector<double> step_ms, job_ms;
+    std::vector<double> step_ms, job_ms, job_step_ms;
     pw_server_stats st{};
+
+    // queued jobs in arrival order that fit the budget, uploaded (beside a
+    // running pass when async) and appended to `into`
+    void admit(bool async, std::vector<int64_t> *into) {
+        for (auto it = queue.begin(); it != queue.end();) {
+            SJob &j = jobs[*it];
+            if (in_use + j.bytes > budget) {
+                ++it;
+                continue;
+            }
+            j.job = eng->add_job(j.id, j.n, j.pos.data(), j.vort.data(), j.rad.data(),
+                                 j.vol.data(), j.par, 0, async);
+            in_use += j.bytes;
+            j.state = JS_RUNNING;
+            j.admitted = now();
+            j.stage = 0;
+            j.stages_left = 4 * j.nsteps;
+            j.pos.clear();
+            j.vort.clear();
+            j.pos.shrink_to_fit();
+            j.vort.shrink_to_fit();
+            st.admitted++;
+            into->push_back(j.id);
+            it = queue.erase(it);
+        }
+    }
 
     double now() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -754,6 +803,8 @@ int pw_server_cancel(pw_server *s, int64_t id) {
         if (j.state == JS_RUNNING) {
             s->eng->remove_job(id);
             s->in_use -= j.bytes;
+            j.job = nullptr;
+            s->resident.erase(std::find(s->resident.begin(), s->resident.end(), id));
         } else if (j.state == JS_QUEUED) {
             s->queue.erase(std::find(s->queue.begin(), s->queue.end(), id));
         } else {
@@ -770,58 +821,70 @@ int pw_server_step(pw_server *s, int64_t *running) {
     if (!s) return PW_EINVAL;
     return guarded(&s->err, [&] {
         const double t_begin = s->now();
-        // admission: queued jobs in arrival order that fit the budget
-        for (auto it = s->queue.begin(); it != s->queue.end();) {
-            SJob &j = s->jobs[*it];
-            if (s->in_use + j.bytes > s->budget) {
-                ++it;
-                continue;
-            }
-            s->eng->add_job(j.id, j.n, j.pos.data(), j.vort.data(), j.rad.data(), j.vol.data(),
-                            j.par);
-            s->in_use += j.bytes;
-            j.state = JS_RUNNING;
-            j.pos.clear();
-            j.vort.clear();
-            s->st.admitted++;
-            it = s->queue.erase(it);
+        // an idle server admits first, so that arrivals start in this pass
+        if (s->resident.empty()) s->admit(false, &s->resident);
+        // the pass: every resident job's stages for it
+        std::vector<pw::PassJob> pj;
+        double ref = kPassFloor;
+        for (int64_t id : s->resident) ref = std::max(ref, stage_cost(s->jobs[id]));
+        for (int64_t id : s->resident) {
+            SJob &j = s->jobs[id];
+            pj.push_back(pw::PassJob{j.job, j.stage, stages_this_pass(j, ref)});
         }
-        // one batched RK4 step over every resident job (captured graph)
-        if (!s->eng->jobs().empty()) {
-            s->eng->rk4(1, true);
+        std::vector<int64_t> joining;
+        std::exception_ptr admit_err;
+        if (!pj.empty()) {
+            s->eng->pass_begin(pj, true);
+            // admission beside the running pass; the new jobs join the next
+            // one.  The pass has run whatever admission does: its stages are
+            // accounted for before an admission error is passed on.
+            try {
+                s->admit(true, &joining);
+            } catch (...) {
+                admit_err = std::current_exception();
+            }
+            std::vector<char> bad;
+            s->eng->pass_end(&bad);
             s->st.steps++;
             s->st.pairs += s->eng->last_pairs;
             s->st.busy_ms += s->eng->last_total_ms;
-        }
-        // retirement
-        std::vector<int64_t> ids;
-        for (auto &jp : s->eng->jobs()) ids.push_back(jp->id);
-        for (int64_t id : ids) {
-            SJob &j = s->jobs[id];
-            Job *dj = s->eng->find(id);
-            bool bad = s->eng->take_flag(dj);
-            j.done++;
-            if (bad || j.done >= j.nsteps) {
-                if (!bad) {
+            // advance, then retire finished and failed jobs
+            std::vector<int64_t> keep;
+            for (size_t k = 0; k < pj.size(); ++k) {
+                const int64_t id = s->resident[k];
+                SJob &j = s->jobs[id];
+                j.stage = (j.stage + pj[k].stages) % 4;
+                j.stages_left -= (size_t)pj[k].stages;
+                j.done = (4 * j.nsteps - j.stages_left) / 4;
+                if (!bad[k] && j.stages_left > 0) {
+                    keep.push_back(id);
+                    continue;
+                }
+                if (!bad[k]) {
                     j.pos.resize(3 * j.n);
                     j.vort.resize(3 * j.n);
-                    s->eng->download_aos(dj, 0, j.pos.data());
-                    s->eng->download_aos(dj, 1, j.vort.data());
+                    s->eng->download_aos(j.job, 0, j.pos.data());
+                    s->eng->download_aos(j.job, 1, j.vort.data());
                 }
                 s->eng->remove_job(id);
+                j.job = nullptr;
                 s->in_use -= j.bytes;
-                j.state = bad ? JS_FAILED : JS_DONE;
+                j.state = bad[k] ? JS_FAILED : JS_DONE;
                 j.finished = s->now();
-                if (bad)
+                if (bad[k]) {
                     s->st.failed++;
-                else {
+                } else {
                     s->st.finished++;
                     s->job_ms.push_back(j.finished - j.arrival);
+                    s->job_step_ms.push_back((j.finished - j.admitted) / (double)j.nsteps);
                 }
             }
+            s->resident.swap(keep);
+            s->step_ms.push_back(s->now() - t_begin);
         }
-        if (!ids.empty()) s->step_ms.push_back(s->now() - t_begin);
-        if (running) *running = (int64_t)s->eng->jobs().size();
+        s->resident.insert(s->resident.end(), joining.begin(), joining.end());
+        if (running) *running = (int64_t)s->resident.size();
+        if (admit_err) std::rethrow_exception(admit_err);
     });
 }
 
@@ -851,11 +914,13 @@ int pw_server_get_stats(pw_server *s, pw_server_stats *st) {
     if (!s || !st) return PW_EINVAL;
     *st = s->st;
     st->queued = (int64_t)s->queue.size();
-    st->running = (int64_t)s->eng->jobs().size();
+    st->running = (int64_t)s->resident.size();
     st->step_p50_ms = pct(s->step_ms, 0.50);
     st->step_p99_ms = pct(s->step_ms, 0.99);
     st->job_p50_ms = pct(s->job_ms, 0.50);
     st->job_p99_ms = pct(s->job_ms, 0.99);
+    st->job_step_p50_ms = pct(s->job_step_ms, 0.50);
+    st->job_step_p99_ms = pct(s->job_step_ms, 0.99);
     st->hbm_in_use = s->in_use;
     st->hbm_budget = s->budget;
     return PW_OK;

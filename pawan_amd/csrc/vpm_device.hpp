@@ -1056,7 +1056,9 @@ __global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ 
 // powers of sigma are already inside and su = sq = sf = 1/kq, sd = 2 nu / kz;
 // kq, kz from pair_scale), then apply the RK4 / Euler stage combination of rk4.cpp:21-66
 // with unfused multiplies and adds in the reference's order, and write the
-// next stage's record.  Block 0 also re-arms the pair kernels' queues.
+// next stage's record.  Block 0 also re-arms the pair kernels' queues.  With
+// mode == MODE_PER_JOB each job runs the stage its JobDev names (a server pass
+// in which jobs are at different RK4 stages).
 template <typename Rec>
 __device__ __forceinline__ void store_rec(Rec *r, int i, const double *xs, double scale);
 template <>
@@ -1157,6 +1159,7 @@ __global__ __launch_bounds__(256) void combine_kernel(const JobDev *__restrict__
         c.tile = __builtin_amdgcn_readfirstlane(c.tile);
         const JobDev &J = jobs[c.job];
         const size_t np = (size_t)J.npad;
+        const int jmode = mode == MODE_PER_JOB ? J.mode : mode;
         if (J.sym) {
             // target side: the chunks from the one holding this particle's
             // block start; source side: the blocks up to this particle's
@@ -1228,7 +1231,7 @@ __global__ __launch_bounds__(256) void combine_kernel(const JobDev *__restrict__
                 k[3] = J.sq * U[3] + J.sd * (vi * U[6] - ax * U[9]);
                 k[4] = J.sq * U[4] + J.sd * (vi * U[7] - ay * U[9]);
                 k[5] = J.sq * U[5] + J.sd * (vi * U[8] - az * U[9]);
-                finish_particle(J, i, k, mode);
+                finish_particle(J, i, k, jmode);
             }
             __syncthreads();  // red is rewritten by the next symmetric item
             continue;
@@ -1252,7 +1255,7 @@ __global__ __launch_bounds__(256) void combine_kernel(const JobDev *__restrict__
         k[3] = J.sq * S[3] + J.sf * S[6] + J.sd * (vi * S[9] - ax * S[12]);
         k[4] = J.sq * S[4] + J.sf * S[7] + J.sd * (vi * S[10] - ay * S[12]);
         k[5] = J.sq * S[5] + J.sf * S[8] + J.sd * (vi * S[11] - az * S[12]);
-        finish_particle(J, i, k, mode);
+        finish_particle(J, i, k, jmode);
     }
 }
 

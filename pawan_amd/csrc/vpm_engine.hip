@@ -58,6 +58,8 @@ void HostBuf::ensure(size_t b) {
 }
 
 static size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+// jobs one engine holds at once (one non-finite flag each)
+constexpr int kMaxJobSlots = 65536;
 static int grid_for(size_t n) { return (int)std::min<size_t>(4096, (n + 255) / 256 + 1); }
 
 size_t chunk_for(size_t n, int TB) {
@@ -145,7 +147,8 @@ JobDev Job::dev() const {
     d.rate = s + 4 * blk;
     d.part = part.as<double>();
     d.vol = vol.as<double>();
-    d.flag = flag.as<int>();
+    d.flag = flag;
+    d.mode = MODE_SOLVE;
     d.n = (int)n;
     d.npad = (int)npad;
     d.nchunk = nchunk;
@@ -286,6 +289,10 @@ Engine::Engine(int device, int kind, const Variant &v)
     gridc_ = 4 * cus_;
     counters_.reset(2 * sizeof(int));
     PW_HIP(hipMemsetAsync(counters_.p, 0, 2 * sizeof(int), stream_));
+    flags_.reset(kMaxJobSlots * sizeof(int));
+    PW_HIP(hipMemsetAsync(flags_.p, 0, flags_.bytes, stream_));
+    flags_host_.ensure(flags_.bytes);
+    for (int k = kMaxJobSlots - 1; k >= 0; --k) free_slots_.push_back(k);
     PW_HIP(hipEventCreate(&ev_begin_));
     PW_HIP(hipEventCreate(&ev_end_));
     PW_HIP(hipStreamSynchronize(stream_));
@@ -296,6 +303,14 @@ Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
+    for (int r = 0; r < 4; ++r) {
+        if (pgexec_[r]) (void)hipGraphExecDestroy(pgexec_[r]);
+        if (pgraph_[r]) (void)hipGraphDestroy(pgraph_[r]);
+    }
+    if (upload_) {
+        (void)hipStreamSynchronize(upload_);
+        (void)hipStreamDestroy(upload_);
+    }
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
     if (ev_begin_) (void)hipEventDestroy(ev_begin_);
     if (ev_end_) (void)hipEventDestroy(ev_end_);
@@ -311,14 +326,23 @@ Job *Engine::find(int64_t id) {
     return nullptr;
 }
 
+hipStream_t Engine::upload_stream() {
+    if (!upload_) PW_HIP(hipStreamCreateWithFlags(&upload_, hipStreamNonBlocking));
+    return upload_;
+}
+
 Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort,
                      const double *radius, const double *volume, const JobParams &par,
-                     size_t chunk_override) {
+                     size_t chunk_override, bool async_upload) {
     if (n == 0 || !pos || !vort || !radius || !volume) throw Error(PW_EINVAL, "empty job or null array");
     if (n > (size_t)INT32_MAX / 2) throw Error(PW_EINVAL, "too many particles");
     if (par.precision != PREC_FP64 && par.precision != PREC_FP32C)
         throw Error(PW_EINVAL, "unknown precision");
+    if (free_slots_.empty()) throw Error(PW_ENOMEM, "too many jobs in one engine");
     PW_HIP(hipSetDevice(device_));
+    // the upload runs on the compute stream, or beside it
+    hipStream_t s = async_upload ? upload_stream() : stream_;
+    DevBuf &dst = async_upload ? ustage_ : dstage_;
     auto j = std::make_unique<Job>();
     j->id = id;
     j->n = n;
@@ -347,10 +371,9 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     j->soa.reset(5 * 6 * np * sizeof(double));
     j->part.reset((size_t)j->nchunk * NCOMP * np * sizeof(double));
     j->vol.reset(2 * np * sizeof(double));  // volume | radius
-    j->flag.reset(256);
     j->tw = tb * tpt;
     alloc_jpart(j.get());
-    j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + j->flag.bytes +
+    j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + sizeof(int) +
                j->jpart.bytes;
     // host-side staging of records, state and constants
     size_t need = std::max(np * rec_bytes(par.precision), 6 * np * sizeof(double));
@@ -358,22 +381,25 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     std::memset(stage_.p, 0, np * rec_bytes(par.precision));
     fill_records(stage_.p, par.precision, n, pos, vort, radius, volume, j->rec_scale());
     PW_HIP(hipMemcpyAsync(j->rec.p, stage_.p, np * rec_bytes(par.precision), hipMemcpyHostToDevice,
-                          stream_));
-    PW_HIP(hipStreamSynchronize(stream_));
+                          s));
+    PW_HIP(hipStreamSynchronize(s));
     double *v = stage_.as<double>();
     std::memset(v, 0, 2 * np * sizeof(double));
     for (size_t i = 0; i < n; ++i) {
         v[i] = volume[i];
         v[np + i] = radius[i];
     }
-    PW_HIP(hipMemcpyAsync(j->vol.p, v, 2 * np * sizeof(double), hipMemcpyHostToDevice, stream_));
-    PW_HIP(hipStreamSynchronize(stream_));
-    PW_HIP(hipMemsetAsync(j->soa.p, 0, j->soa.bytes, stream_));
-    PW_HIP(hipMemsetAsync(j->flag.p, 0, j->flag.bytes, stream_));
+    PW_HIP(hipMemcpyAsync(j->vol.p, v, 2 * np * sizeof(double), hipMemcpyHostToDevice, s));
+    PW_HIP(hipStreamSynchronize(s));
+    PW_HIP(hipMemsetAsync(j->soa.p, 0, j->soa.bytes, s));
+    j->slot = free_slots_.back();
+    j->flag = flags_.as<int>() + j->slot;
+    PW_HIP(hipMemsetAsync(j->flag, 0, sizeof(int), s));
+    upload_soa(j.get(), pos, vort, s, dst);
+    free_slots_.pop_back();
     Job *raw = j.get();
     jobs_.push_back(std::move(j));
     plan_dirty_ = true;
-    upload_state_soa(raw, pos, vort);
     return raw;
 }
 
@@ -460,7 +486,7 @@ void Engine::append(Job *j, size_t m, const double *pos, const double *vort, con
         j->part.reset((size_t)j->part_chunks * NCOMP * np * sizeof(double));
     }
     alloc_jpart(j);
-    j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + j->flag.bytes +
+    j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + sizeof(int) +
                j->jpart.bytes;
     plan_dirty_ = true;
 }
@@ -480,6 +506,7 @@ void Engine::remove_job(int64_t id) {
     PW_HIP(hipStreamSynchronize(stream_));
     for (size_t k = 0; k < jobs_.size(); ++k)
         if (jobs_[k]->id == id) {
+            free_slots_.push_back(jobs_[k]->slot);
             jobs_.erase(jobs_.begin() + k);
             plan_dirty_ = true;
             return;
@@ -493,6 +520,62 @@ void Engine::set_params(Job *j, const JobParams &par) {
     plan_dirty_ = true;
 }
 
+// Work items of job j (table index k): pair items into dst, combine items
+// into ci.
+static void job_items(const Job &j, int k, std::vector<Item> &dst, std::vector<CItem> &ci) {
+    if (!j.sym) {
+        for (int c = 0; c < j.nchunk; ++c)
+            for (int t = 0; t < j.ntile; ++t) dst.push_back(Item{k, t, c, j.chunk_len});
+    } else {
+        // block t pairs with the sources from its own start on: chunks
+        // from the one holding that start; cost = sources visited
+        for (int t = 0; t < j.ntile; ++t) {
+            const size_t b0 = (size_t)t * j.tw;
+            for (int c = (int)(b0 / j.chunk_len); c < j.nchunk; ++c) {
+                const size_t lo = std::max(b0, (size_t)c * j.chunk_len);
+                const size_t hi = std::min(j.n, (size_t)(c + 1) * j.chunk_len);
+                dst.push_back(Item{k, t, c, (int)(hi - lo)});
+            }
+        }
+    }
+    if (j.sym) {  // 64 particles per item
+        for (int t = 0; t < (int)((j.n + 63) / 64); ++t) ci.push_back(CItem{k, t});
+    } else {
+        for (int t = 0; t < (int)((j.n + 255) / 256); ++t) ci.push_back(CItem{k, t});
+    }
+}
+
+// larger items first (their cost is tile x chunk length): shorter tail
+static void order_pair_items(std::vector<Item> &v) {
+    std::stable_sort(v.begin(), v.end(), [](const Item &a, const Item &b) { return a.pad > b.pad; });
+}
+
+// Combine items: the combine grid takes item k, k + G, k + 2G, ... per block
+// (G blocks, all resident); sorted by the values each reads, largest first,
+// and every second round of G reversed, so that each block's items add up to
+// about the same work.  tab[c.job] is the item's job.
+static void order_combine_items(std::vector<CItem> &ci, const std::vector<const Job *> &tab,
+                                size_t G) {
+    auto cost = [&](const CItem &c) -> size_t {
+        const Job &j = *tab[(size_t)c.job];
+        if (!j.sym) return (size_t)NCOMP * 256 * (size_t)j.nchunk;
+        const size_t b0 = (size_t)c.tile * 64 / (size_t)j.tw;
+        const size_t cf = b0 * (size_t)j.tw / (size_t)j.chunk_len;
+        return (size_t)NCOMP_S * 64 * (b0 + 1 + (size_t)j.nchunk - cf);
+    };
+    std::vector<std::pair<size_t, CItem>> kc;
+    kc.reserve(ci.size());
+    for (const CItem &c : ci) kc.push_back({cost(c), c});
+    std::stable_sort(kc.begin(), kc.end(),
+                     [](const std::pair<size_t, CItem> &a, const std::pair<size_t, CItem> &b) {
+                         return a.first > b.first;
+                     });
+    G = std::max<size_t>(1, G);
+    for (size_t r = 0; r * G < kc.size(); ++r)
+        if (r & 1) std::reverse(kc.begin() + r * G, kc.begin() + std::min(kc.size(), (r + 1) * G));
+    for (size_t k = 0; k < ci.size(); ++k) ci[k] = kc[k].second;
+}
+
 void Engine::build_plan() {
     PW_HIP(hipSetDevice(device_));
     // queued kernels may still read the current tables and their host copy
@@ -500,59 +583,16 @@ void Engine::build_plan() {
     std::vector<JobDev> jd;
     std::vector<Item> it64, it32;
     std::vector<CItem> ci;
+    std::vector<const Job *> tab;
     for (size_t k = 0; k < jobs_.size(); ++k) {
         const Job &j = *jobs_[k];
         jd.push_back(j.dev());
-        auto &dst = j.par.precision == PREC_FP64 ? it64 : it32;
-        if (!j.sym) {
-            for (int c = 0; c < j.nchunk; ++c)
-                for (int t = 0; t < j.ntile; ++t) dst.push_back(Item{(int)k, t, c, j.chunk_len});
-        } else {
-            // block t pairs with the sources from its own start on: chunks
-            // from the one holding that start; cost = sources visited
-            for (int t = 0; t < j.ntile; ++t) {
-                const size_t b0 = (size_t)t * j.tw;
-                for (int c = (int)(b0 / j.chunk_len); c < j.nchunk; ++c) {
-                    const size_t lo = std::max(b0, (size_t)c * j.chunk_len);
-                    const size_t hi = std::min(j.n, (size_t)(c + 1) * j.chunk_len);
-                    dst.push_back(Item{(int)k, t, c, (int)(hi - lo)});
-                }
-            }
-        }
-        if (j.sym) {  // 64 particles per item
-            for (int t = 0; t < (int)((j.n + 63) / 64); ++t) ci.push_back(CItem{(int)k, t});
-        } else {
-            for (int t = 0; t < (int)((j.n + 255) / 256); ++t) ci.push_back(CItem{(int)k, t});
-        }
+        tab.push_back(&j);
+        job_items(j, (int)k, j.par.precision == PREC_FP64 ? it64 : it32, ci);
     }
-    // larger items first (their cost is tile x chunk length): shorter tail
-    auto bigger = [](const Item &a, const Item &b) { return a.pad > b.pad; };
-    std::stable_sort(it64.begin(), it64.end(), bigger);
-    std::stable_sort(it32.begin(), it32.end(), bigger);
-    // Combine items: the combine grid takes item k, k + G, k + 2G, ... per
-    // block (G = gridc_ blocks, all resident); sorted by the values each
-    // reads, largest first, and every second round of G reversed, so that
-    // each block's items add up to about the same work.
-    {
-        auto cost = [&](const CItem &c) -> size_t {
-            const Job &j = *jobs_[(size_t)c.job];
-            if (!j.sym) return (size_t)NCOMP * 256 * (size_t)j.nchunk;
-            const size_t b0 = (size_t)c.tile * 64 / (size_t)j.tw;
-            const size_t cf = b0 * (size_t)j.tw / (size_t)j.chunk_len;
-            return (size_t)NCOMP_S * 64 * (b0 + 1 + (size_t)j.nchunk - cf);
-        };
-        std::vector<std::pair<size_t, CItem>> kc;
-        kc.reserve(ci.size());
-        for (const CItem &c : ci) kc.push_back({cost(c), c});
-        std::stable_sort(kc.begin(), kc.end(),
-                         [](const std::pair<size_t, CItem> &a, const std::pair<size_t, CItem> &b) {
-                             return a.first > b.first;
-                         });
-        const size_t G = (size_t)std::max(1, gridc_);
-        for (size_t r = 0; r * G < kc.size(); ++r)
-            if (r & 1) std::reverse(kc.begin() + r * G, kc.begin() + std::min(kc.size(), (r + 1) * G));
-        for (size_t k = 0; k < ci.size(); ++k) ci[k] = kc[k].second;
-    }
+    order_pair_items(it64);
+    order_pair_items(it32);
+    order_combine_items(ci, tab, (size_t)gridc_);
     n64_ = (int)it64.size();
     n32_ = (int)it32.size();
     nci_ = (int)ci.size();
@@ -732,27 +772,184 @@ void Engine::euler(size_t nsteps) {
 }
 
 
+// ------------------------------------------------------------ server pass
+void Engine::launch_round(int r, bool timed) {
+    char *d = pplan_.as<char>() + (size_t)r * p_round_;
+    const JobDev *jd = (const JobDev *)d;
+    const int *cnt = (const int *)(d + po_cnt_);
+    int *q = counters_.as<int>();
+    const bool graph = !timed;
+    if (graph || pn64_[r] > 0) {
+        hipLaunchKernelGGL(fn64_, dim3(grid64_), dim3(tb_), 0, stream_, jd,
+                           (const Item *)(d + po_i64_), cnt + 0, q + 0);
+        PW_HIP(hipGetLastError());
+    }
+    if (graph || pn32_[r] > 0) {
+        hipLaunchKernelGGL(fn32_, dim3(grid32_), dim3(tb32_), 0, stream_, jd,
+                           (const Item *)(d + po_i32_), cnt + 1, q + 1);
+        PW_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(combine_kernel, dim3(gridc_), dim3(256), 0, stream_, jd,
+                       (const CItem *)(d + po_ci_), cnt + 2, q, (int)MODE_PER_JOB);
+    PW_HIP(hipGetLastError());
+}
+
+void Engine::capture_pass(int rounds) {
+    const int g = rounds - 1;
+    if (pgexec_[g]) {
+        (void)hipGraphExecDestroy(pgexec_[g]);
+        pgexec_[g] = nullptr;
+    }
+    if (pgraph_[g]) {
+        (void)hipGraphDestroy(pgraph_[g]);
+        pgraph_[g] = nullptr;
+    }
+    PW_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int r = 0; r < rounds; ++r) launch_round(r, false);
+    } catch (...) {
+        hipGraph_t gr = nullptr;
+        (void)hipStreamEndCapture(stream_, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+        throw;
+    }
+    PW_HIP(hipStreamEndCapture(stream_, &pgraph_[g]));
+    PW_HIP(hipGraphInstantiate(&pgexec_[g], pgraph_[g], nullptr, nullptr, 0));
+    pgraph_layout_[g] = playout_;
+}
+
+void Engine::pass_begin(const std::vector<PassJob> &pj, bool use_graph) {
+    PW_HIP(hipSetDevice(device_));
+    cur_pass_.clear();
+    if (pj.empty()) return;
+    int R = 0;
+    for (const PassJob &p : pj) {
+        if (!p.job || p.stages < 1 || p.first_stage < 0 || p.first_stage + p.stages > 4)
+            throw Error(PW_EINVAL, "a pass runs 1 to 4 stages of a job within one step");
+        R = std::max(R, p.stages);
+    }
+    // the rounds' work: job k runs in rounds 0 .. stages - 1
+    std::vector<const Job *> tab;
+    for (const PassJob &p : pj) tab.push_back(p.job);
+    std::vector<std::vector<Item>> r64(R), r32(R);
+    std::vector<std::vector<CItem>> rci(R);
+    size_t m64 = 0, m32 = 0, mci = 0;
+    for (int r = 0; r < R; ++r) {
+        for (size_t k = 0; k < pj.size(); ++k)
+            if (pj[k].stages > r)
+                job_items(*pj[k].job, (int)k, pj[k].job->par.precision == PREC_FP64 ? r64[r] : r32[r],
+                          rci[r]);
+        order_pair_items(r64[r]);
+        order_pair_items(r32[r]);
+        order_combine_items(rci[r], tab, (size_t)gridc_);
+        m64 = std::max(m64, r64[r].size());
+        m32 = std::max(m32, r32[r].size());
+        mci = std::max(mci, rci[r].size());
+    }
+    // regions only grow: a growth is a new layout, for which the pass graphs
+    // are captured again
+    if (!pplan_.p || pj.size() > pcap_jobs_ || m64 > pcap_i64_ || m32 > pcap_i32_ || mci > pcap_ci_) {
+        pcap_jobs_ = std::max(pcap_jobs_, round_up(pj.size() * 2, 16));
+        pcap_i64_ = std::max(pcap_i64_, round_up(std::max<size_t>(m64, 1) * 2, 1024));
+        pcap_i32_ = std::max(pcap_i32_, round_up(std::max<size_t>(m32, 1) * 2, 1024));
+        pcap_ci_ = std::max(pcap_ci_, round_up(std::max<size_t>(mci, 1) * 2, 1024));
+        po_i64_ = round_up(pcap_jobs_ * sizeof(JobDev), 256);
+        po_i32_ = round_up(po_i64_ + pcap_i64_ * sizeof(Item), 256);
+        po_ci_ = round_up(po_i32_ + pcap_i32_ * sizeof(Item), 256);
+        po_cnt_ = round_up(po_ci_ + pcap_ci_ * sizeof(CItem), 256);
+        p_round_ = round_up(po_cnt_ + 256, 256);
+        PW_HIP(hipStreamSynchronize(stream_));
+        pplan_.reset(4 * p_round_);
+        pplan_host_.ensure(pplan_.bytes);
+        ++playout_;
+    }
+    char *h = pplan_host_.as<char>();
+    double pairs = 0.0;
+    int hi = 0;
+    for (int r = 0; r < R; ++r) {
+        char *b = h + (size_t)r * p_round_;
+        JobDev *jd = (JobDev *)b;
+        for (size_t k = 0; k < pj.size(); ++k) {
+            jd[k] = pj[k].job->dev();
+            jd[k].mode = pj[k].stages > r ? MODE_RK4_1 + pj[k].first_stage + r : MODE_SOLVE;
+        }
+        std::memcpy(b + po_i64_, r64[r].data(), r64[r].size() * sizeof(Item));
+        std::memcpy(b + po_i32_, r32[r].data(), r32[r].size() * sizeof(Item));
+        std::memcpy(b + po_ci_, rci[r].data(), rci[r].size() * sizeof(CItem));
+        const int cnt[4] = {(int)r64[r].size(), (int)r32[r].size(), (int)rci[r].size(), 0};
+        std::memcpy(b + po_cnt_, cnt, sizeof cnt);
+        pn64_[r] = cnt[0];
+        pn32_[r] = cnt[1];
+    }
+    for (const PassJob &p : pj) {
+        pairs += (double)p.stages * (double)p.job->n * (double)(p.job->n - 1);
+        hi = std::max(hi, p.job->slot + 1);
+    }
+    PW_HIP(hipMemcpyAsync(pplan_.p, h, (size_t)R * p_round_, hipMemcpyHostToDevice, stream_));
+    PW_HIP(hipEventRecord(ev_begin_, stream_));
+    if (use_graph) {
+        if (!pgexec_[R - 1] || pgraph_layout_[R - 1] != playout_) capture_pass(R);
+        PW_HIP(hipGraphLaunch(pgexec_[R - 1], stream_));
+    } else {
+        for (int r = 0; r < R; ++r) launch_round(r, true);
+    }
+    // every listed job's flag, in one copy
+    PW_HIP(hipMemcpyAsync(flags_host_.p, flags_.p, (size_t)hi * sizeof(int), hipMemcpyDeviceToHost,
+                          stream_));
+    PW_HIP(hipEventRecord(ev_end_, stream_));
+    cur_pass_ = pj;
+    flag_hi_ = hi;
+    pass_pairs_ = pairs;
+}
+
+void Engine::pass_end(std::vector<char> *nonfinite) {
+    PW_HIP(hipSetDevice(device_));
+    nonfinite->assign(cur_pass_.size(), 0);
+    if (cur_pass_.empty()) {
+        last_total_ms = last_pair_ms = last_pairs = 0.0;
+        return;
+    }
+    PW_HIP(hipEventSynchronize(ev_end_));
+    float ms = 0.f;
+    PW_HIP(hipEventElapsedTime(&ms, ev_begin_, ev_end_));
+    last_total_ms = ms;
+    last_pair_ms = 0.0;
+    last_pairs = pass_pairs_;
+    const int *h = flags_host_.as<int>();
+    bool any = false;
+    for (size_t k = 0; k < cur_pass_.size(); ++k) {
+        (*nonfinite)[k] = h[cur_pass_[k].job->slot] != 0;
+        any = any || (*nonfinite)[k];
+    }
+    if (any) PW_HIP(hipMemsetAsync(flags_.p, 0, (size_t)flag_hi_ * sizeof(int), stream_));
+    cur_pass_.clear();
+}
+
 void Engine::upload_state_soa(Job *j, const double *pos, const double *vort) {
     PW_HIP(hipSetDevice(device_));
+    upload_soa(j, pos, vort, stream_, dstage_);
+}
+
+void Engine::upload_soa(Job *j, const double *pos, const double *vort, hipStream_t s,
+                        DevBuf &stage) {
     const size_t np = j->npad, n = j->n;
-    if (dstage_.bytes < 3 * n * sizeof(double)) {
-        PW_HIP(hipStreamSynchronize(stream_));
-        dstage_.reset(round_up(6 * np * sizeof(double), 4096));
+    if (stage.bytes < 3 * n * sizeof(double)) {
+        PW_HIP(hipStreamSynchronize(s));
+        stage.reset(round_up(6 * np * sizeof(double), 4096));
     }
     JobDev d = j->dev();
     const double *src[2] = {pos, vort};
     for (int h = 0; h < 2; ++h) {
-        PW_HIP(hipMemcpyAsync(dstage_.p, src[h], 3 * n * sizeof(double), hipMemcpyHostToDevice,
-                              stream_));
-        hipLaunchKernelGGL(aos_soa_kernel, dim3(grid_for(n)), dim3(256), 0, stream_,
-                           dstage_.as<double>(), d.x0 + (size_t)3 * h * np, (int)n, (int)np, 0);
+        PW_HIP(hipMemcpyAsync(stage.p, src[h], 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(aos_soa_kernel, dim3(grid_for(n)), dim3(256), 0, s, stage.as<double>(),
+                           d.x0 + (size_t)3 * h * np, (int)n, (int)np, 0);
         PW_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(copy_soa_kernel, dim3(grid_for(6 * np)), dim3(256), 0, stream_,
+    hipLaunchKernelGGL(copy_soa_kernel, dim3(grid_for(6 * np)), dim3(256), 0, s,
                        (const double *)d.x0, d.xs, 6 * np);
-    hipLaunchKernelGGL(refresh_rec_kernel, dim3((n + 255) / 256), dim3(256), 0, stream_, d);
+    hipLaunchKernelGGL(refresh_rec_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d);
     PW_HIP(hipGetLastError());
-    PW_HIP(hipStreamSynchronize(stream_));
+    PW_HIP(hipStreamSynchronize(s));
 }
 
 void Engine::download_aos(Job *j, int which, double *out) { download_aos(j, which, out, 0, j->n); }
@@ -920,10 +1117,31 @@ void Engine::influence(Job *j, double *field_aos) {
 bool Engine::take_flag(Job *j) {
     PW_HIP(hipSetDevice(device_));
     int f = 0;
-    PW_HIP(hipMemcpyAsync(&f, j->flag.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
-    PW_HIP(hipMemsetAsync(j->flag.p, 0, sizeof(int), stream_));
+    PW_HIP(hipMemcpyAsync(&f, j->flag, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    PW_HIP(hipMemsetAsync(j->flag, 0, sizeof(int), stream_));
     PW_HIP(hipStreamSynchronize(stream_));
     return f != 0;
+}
+
+std::vector<char> Engine::take_flags() {
+    PW_HIP(hipSetDevice(device_));
+    std::vector<char> out(jobs_.size(), 0);
+    int hi = 0;
+    for (auto &j : jobs_) hi = std::max(hi, j->slot + 1);
+    if (hi == 0) return out;
+    int *h = flags_host_.as<int>();
+    PW_HIP(hipMemcpyAsync(h, flags_.p, (size_t)hi * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    PW_HIP(hipStreamSynchronize(stream_));
+    bool any = false;
+    for (size_t k = 0; k < jobs_.size(); ++k) {
+        out[k] = h[jobs_[k]->slot] != 0;
+        any = any || out[k];
+    }
+    if (any) {
+        PW_HIP(hipMemsetAsync(flags_.p, 0, (size_t)hi * sizeof(int), stream_));
+        PW_HIP(hipStreamSynchronize(stream_));
+    }
+    return out;
 }
 
 

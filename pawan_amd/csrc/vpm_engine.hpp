@@ -93,7 +93,9 @@ struct Job {
     int kind = 0;  // smoothing kernel of the engine that owns the job
     bool uniform = false;
     double sigma = 0.0;
-    DevBuf rec, soa, part, vol, flag;  // soa holds x0, xs, k1, k23, rate
+    DevBuf rec, soa, part, vol;  // soa holds x0, xs, k1, k23, rate
+    int slot = -1;               // the job's entry in the engine's flag array
+    int *flag = nullptr;         // device address of that entry
     DevBuf jpart;               // source-side slots of the symmetric pass
     bool sym = false;           // binary32 job on the symmetric pass
     int tw = 0;                 // targets per block of the pass
@@ -126,6 +128,13 @@ void rates_noalloc(int kind, double nu, const double uinf[3], size_t n, const do
 size_t noalloc_max_n();
 size_t noalloc_scratch_bytes();
 
+// One job's share of a server pass: RK4 stages first_stage .. first_stage +
+// stages - 1 (0-based, within one step).
+struct PassJob {
+    Job *job;
+    int first_stage, stages;
+};
+
 class Engine {
   public:
     Engine(int device, int kind, const Variant &v);
@@ -141,9 +150,12 @@ class Engine {
 
     // Create a job from N x 3 positions/vorticities and N radius/volume.
     // chunk_override > 0 fixes the source chunk length (multiple of block).
+    // async_upload: copy and initialise on the engine's upload stream, so
+    // that kernels already queued on the compute stream keep running (the
+    // call returns once the job is ready).
     Job *add_job(int64_t id, size_t n, const double *pos, const double *vort,
                  const double *radius, const double *volume, const JobParams &par,
-                 size_t chunk_override = 0);
+                 size_t chunk_override = 0, bool async_upload = false);
     void remove_job(int64_t id);
     // Append m particles to a job (particle emission), between steps.  The
     // device arrays grow geometrically; existing state is kept on the device.
@@ -163,6 +175,8 @@ class Engine {
 
     // Host <-> device transfers (synchronous on the engine stream).
     void upload_state_soa(Job *j, const double *pos, const double *vort);  // N x 3 each
+    // the same on stream s through the device staging buffer `stage`
+    void upload_soa(Job *j, const double *pos, const double *vort, hipStream_t s, DevBuf &stage);
     void download_aos(Job *j, int which, double *out);  // which: 0 pos 1 vort 2 vel 3 dvort
     // particles [first, first + count) only
     void download_aos(Job *j, int which, double *out, size_t first, size_t count);
@@ -178,8 +192,18 @@ class Engine {
     void diagnostics(Job *j, Diagnostics *out, bool quadratic);
     // vorticity field at every particle (influence(), interaction.cpp:110-155), N x 3
     void influence(Job *j, double *field_aos);
-    // non-finite flags, read back and cleared
+    // non-finite flags, read back and cleared: one job's, or every job's in
+    // jobs() order with one copy
     bool take_flag(Job *j);
+    std::vector<char> take_flags();
+
+    // A server pass: every listed job runs its stages, the jobs running
+    // round r (0-based) being those with stages > r, each round one pair
+    // launch per precision and one combine.  pass_begin queues the pass (and
+    // a copy of the flags) and returns; pass_end waits for it and gives each
+    // listed job's non-finite flag, cleared.  Timing as for rk4().
+    void pass_begin(const std::vector<PassJob> &pj, bool use_graph);
+    void pass_end(std::vector<char> *nonfinite);
 
     void sync();
 
@@ -189,6 +213,9 @@ class Engine {
 
   private:
     void build_plan();
+    void capture_pass(int rounds);
+    void launch_round(int r, bool timed);
+    hipStream_t upload_stream();
     void launch_stage(int mode, bool timed);
     void capture_graph();
     double pairs_per_rhs() const;
@@ -226,6 +253,29 @@ class Engine {
     hipEvent_t ev_begin_ = nullptr, ev_end_ = nullptr;
     HostBuf stage_;
     DevBuf dstage_;
+    // non-finite flags: one int per job slot
+    DevBuf flags_;
+    HostBuf flags_host_;
+    std::vector<int> free_slots_;
+    // uploads of jobs admitted while a pass runs
+    hipStream_t upload_ = nullptr;
+    DevBuf ustage_;
+    // server pass plan: per round a JobDev table (with the round's stage per
+    // job), pair items per precision, combine items and counts; regions only
+    // grow (graphs hold their addresses)
+    DevBuf pplan_;
+    HostBuf pplan_host_;
+    size_t pcap_jobs_ = 0, pcap_i64_ = 0, pcap_i32_ = 0, pcap_ci_ = 0;
+    size_t p_round_ = 0, po_i64_ = 0, po_i32_ = 0, po_ci_ = 0, po_cnt_ = 0;
+    uint64_t playout_ = 0;
+    hipGraph_t pgraph_[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t pgexec_[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint64_t pgraph_layout_[4] = {~0ull, ~0ull, ~0ull, ~0ull};
+    int pn64_[4] = {0, 0, 0, 0}, pn32_[4] = {0, 0, 0, 0};
+    std::vector<PassJob> cur_pass_;
+    int flag_hi_ = 0;  // flag entries copied by the last pass
+    double pass_pairs_ = 0.0;
+    bool pass_timed_ = false;
 };
 
 }  // namespace pw

@@ -20,7 +20,8 @@ constexpr int NCOMP = 13;
 // (slots per target block; the stretching there enters with a minus sign).
 constexpr int NCOMP_S = 10;
 
-// Combine-kernel modes
+// Combine-kernel modes (MODE_PER_JOB: each job's JobDev::mode)
+constexpr int MODE_PER_JOB = -1;
 constexpr int MODE_SOLVE = 0;
 constexpr int MODE_RK4_1 = 1;
 constexpr int MODE_RK4_2 = 2;
@@ -89,6 +90,7 @@ struct JobDev {
     int *flag;      // set to 1 when a rate or state is non-finite
     int n, npad, nchunk, chunk_len;
     int prec, uniform;
+    int mode;       // stage this job runs when the combine is launched with MODE_PER_JOB
     int sym;        // binary32 symmetric pass: part holds NCOMP_S target-side
                     // components, jpart the source-side ones
     int tw;         // targets per block of the symmetric pass

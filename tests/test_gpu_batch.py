@@ -141,3 +141,47 @@ def test_gaussian_batch_default_variant():
         print(f"  job {k} prec {prec}: pos {ep:.3e} vort {ea:.3e}", flush=True)
         assert ep < (1e-11 if prec == pa.FP64 else 1e-6)
         assert ea < (1e-9 if prec == pa.FP64 else 1e-5)
+
+
+def test_server_passes_by_stage():
+    """A pass runs 4 RK4 stages of a job whose stage costs at most a quarter
+    of the largest resident stage, and 1 stage of the largest: with a
+    65,536-particle rotor wake resident, a 2,048-particle ring completes a
+    step per pass while the wake completes one every 4 passes.  A cancel in
+    the middle of the wake's step, and both results, are as when alone."""
+    big = gen.rotor_wake(256)                          # 65,536
+    small = gen.vortex_ring(128, 16, seed=4)           # 2,048
+    mid = gen.vortex_ring(160, 64, R=1.0, core=0.1, seed=6)  # 10,240
+    print("case server passes by stage: 65,536 + 2,048 + 10,240 (cancelled)", flush=True)
+    alone = {}
+    for name, w, nu, dt, steps in (("big", big, 1e-4, 1e-5, 2), ("small", small, 1e-3, 0.005, 3)):
+        srv = pa.Server(64 << 30)
+        jid = srv.submit(w, nu, dt, steps)
+        while srv.step() > 0:
+            pass
+        alone[name] = srv.fetch(jid)
+        srv.close()
+    srv = pa.Server(64 << 30)
+    jb = srv.submit(big, 1e-4, 1e-5, 2)
+    js = srv.submit(small, 1e-3, 0.005, 3)
+    jm = srv.submit(mid, 1e-3, 0.005, 50)
+    for p in range(3):
+        srv.step()
+        assert srv.state(js)[1] == p + 1  # a whole step per pass
+        assert srv.state(jb)[1] == 0      # one stage per pass
+    assert srv.state(js)[0] == pa.Server.DONE
+    srv.cancel(jm)  # in the middle of its run
+    passes = 3
+    while True:
+        passes += 1
+        if srv.step() == 0:
+            break
+    assert passes == 8  # 2 steps x 4 stages of the wake
+    for name, jid in (("big", jb), ("small", js)):
+        pos, vort = srv.fetch(jid)
+        assert np.array_equal(pos, alone[name][0]) and np.array_equal(vort, alone[name][1]), name
+    st = srv.stats()
+    print(f"  passes {st['steps']}, step p50/p99 {st['step_p50_ms']:.2f}/{st['step_p99_ms']:.2f} ms, "
+          f"job step p50 {st['job_step_p50_ms']:.2f} ms", flush=True)
+    assert st["hbm_in_use"] == 0 and st["cancelled"] == 1
+    srv.close()
