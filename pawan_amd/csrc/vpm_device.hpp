@@ -652,17 +652,32 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
     constexpr int SBJ = PW_SYM_RUN / TPT;  // steps per source-side run (same term count)
     __shared__ SrcTile tile;
     __shared__ double jl[NW][NCOMP_S][64];  // per-source binary64 sums, per wave
+    __shared__ double prev[NCOMP_S][64];    // the strip row's sums before this block
     __shared__ int slot;
     const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int nitems = *nitems_p;
     for (;;) {
-        const int it = next_item(counter, &slot);
+        // uniform: the item's fields and its job's are scalar loads
+        const int it = __builtin_amdgcn_readfirstlane(next_item(counter, &slot));
         if (it >= nitems) break;
         const Item item = items[it];
         const JobDev &J = jobs[item.job];
         const Rec32 *rec = (const Rec32 *)J.rec;
         const int n = J.n;
-        const int B0 = item.tile * TW;
+        // the item is one strip of blocks against one source chunk; the
+        // strip's source-side sums go to one slot row, the blocks adding in
+        // order into what the first block wrote
+        const int bper = J.sw / TW, nblk = (n + TW - 1) / TW;
+        const int b_lo = item.tile * bper, b_hi = min(b_lo + bper, nblk);
+        const int S0 = item.tile * J.sw;
+        const size_t len = (size_t)J.npad - (size_t)S0;  // this strip's slot row length
+        double *jrow = J.jpart + jpart_offset((size_t)item.tile, (size_t)J.npad, (size_t)J.sw);
+        for (int blk = b_lo; blk < b_hi; ++blk) {
+        const int B0 = blk * TW;
+        const int c0 = max(item.chunk * J.chunk_len, B0);
+        const int c1 = min(item.chunk * J.chunk_len + J.chunk_len, n);
+        if (c0 >= c1) break;  // later blocks start later still
+        const bool first = blk == b_lo;
         int idx[TPT], wt[TPT], nt[TPT];
         Part32 me[TPT];
         double isum[TPT][NCOMP_S];  // target-side sums: 32-term binary32 runs added in binary64
@@ -677,10 +692,6 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
             for (int m = 0; m < NCOMP_S; ++m) isum[p][m] = 0.0;
         }
         const bool uniform = J.uniform != 0;
-        const int c0 = max(item.chunk * J.chunk_len, B0);
-        const int c1 = min(item.chunk * J.chunk_len + J.chunk_len, n);
-        const size_t len = (size_t)J.npad - (size_t)B0;  // this block's slot row length
-        double *jrow = J.jpart + jpart_offset((size_t)item.tile, (size_t)J.npad, (size_t)TW);
         for (int s0 = c0; s0 < c1; s0 += 64) {
             const int cnt = min(64, c1 - s0);
             const int st = s0 >> 6;  // source tile
@@ -703,6 +714,12 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                     tile.h[k + c] = (float)r.h;
                 }
             }
+            if (!first)  // the row's sums so far, loaded beside the tile
+                for (int e = (NW > 1 ? (int)threadIdx.x - 64 : (int)threadIdx.x); e >= 0 && e < NCOMP_S * 64;
+                     e += (NW > 1 ? TB - 64 : TB)) {
+                    const int m = e >> 6, j = e & 63;
+                    if (j < cnt) prev[m][j] = jrow[(size_t)m * len + (size_t)(s0 - S0 + j)];
+                }
             __syncthreads();
             bool live[TPT], diag[TPT], full = cnt == 64, any = false;
 #pragma unroll
@@ -774,17 +791,19 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                     double v = jl[0][m][j];
 #pragma unroll
                     for (int ww = 1; ww < NW; ++ww) v += jl[ww][m][j];
-                    jrow[(size_t)m * len + (size_t)(s0 - B0 + j)] = v;
+                    jrow[(size_t)m * len + (size_t)(s0 - S0 + j)] = first ? v : prev[m][j] + v;
                 }
             }
         }
+        const size_t pl = sym_part_len((size_t)item.chunk, (size_t)J.npad, (size_t)J.chunk_len, (size_t)TW);
+        double *orow = J.part + sym_part_offset((size_t)item.chunk, (size_t)J.npad, (size_t)J.chunk_len, (size_t)TW);
 #pragma unroll
         for (int p = 0; p < TPT; ++p)
             if (idx[p] < n) {
-                double *o = J.part + (size_t)item.chunk * NCOMP_S * J.npad + idx[p];
 #pragma unroll
-                for (int m = 0; m < NCOMP_S; ++m) o[(size_t)m * J.npad] = isum[p][m];
+                for (int m = 0; m < NCOMP_S; ++m) orow[(size_t)m * pl + idx[p]] = isum[p][m];
             }
+        }  // blocks of the strip
     }
 }
 
@@ -940,17 +959,30 @@ __global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ 
     constexpr int SB = PW_SYM64_UNROLL;  // unrolled steps
     __shared__ SrcTile64 tile;
     __shared__ double jl[NW][NCOMP_S][64];
+    __shared__ double prev[NCOMP_S][64];  // the strip row's sums before this block
     __shared__ int slot;
     const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int nitems = *nitems_p;
     for (;;) {
-        const int it = next_item(counter, &slot);
+        // uniform: the item's fields and its job's are scalar loads
+        const int it = __builtin_amdgcn_readfirstlane(next_item(counter, &slot));
         if (it >= nitems) break;
         const Item item = items[it];
         const JobDev &J = jobs[item.job];
         const Rec64 *rec = (const Rec64 *)J.rec;
         const int n = J.n;
-        const int B0 = item.tile * TW;
+        // one strip of blocks against one source chunk (see pair32s_kernel)
+        const int bper = J.sw / TW, nblk = (n + TW - 1) / TW;
+        const int b_lo = item.tile * bper, b_hi = min(b_lo + bper, nblk);
+        const int S0 = item.tile * J.sw;
+        const size_t len = (size_t)J.npad - (size_t)S0;
+        double *jrow = J.jpart + jpart_offset((size_t)item.tile, (size_t)J.npad, (size_t)J.sw);
+        for (int blk = b_lo; blk < b_hi; ++blk) {
+        const int B0 = blk * TW;
+        const int c0 = max(item.chunk * J.chunk_len, B0);
+        const int c1 = min(item.chunk * J.chunk_len + J.chunk_len, n);
+        if (c0 >= c1) break;
+        const bool first = blk == b_lo;
         int idx[TPT], wt[TPT], nt[TPT];
         Part64 me[TPT];
         Acc64s a[TPT];
@@ -965,10 +997,6 @@ __global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ 
             for (int m = 0; m < NCOMP_S; ++m) a[p].v[m] = 0.0;
         }
         const bool uniform = J.uniform != 0;
-        const int c0 = max(item.chunk * J.chunk_len, B0);
-        const int c1 = min(item.chunk * J.chunk_len + J.chunk_len, n);
-        const size_t len = (size_t)J.npad - (size_t)B0;
-        double *jrow = J.jpart + jpart_offset((size_t)item.tile, (size_t)J.npad, (size_t)TW);
         for (int s0 = c0; s0 < c1; s0 += 64) {
             const int cnt = min(64, c1 - s0);
             const int st = s0 >> 6;
@@ -990,6 +1018,12 @@ __global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ 
                     tile.h[k + c] = r.h;
                 }
             }
+            if (!first)  // the row's sums so far, loaded beside the tile
+                for (int e = (NW > 1 ? (int)threadIdx.x - 64 : (int)threadIdx.x); e >= 0 && e < NCOMP_S * 64;
+                     e += (NW > 1 ? TB - 64 : TB)) {
+                    const int m = e >> 6, j = e & 63;
+                    if (j < cnt) prev[m][j] = jrow[(size_t)m * len + (size_t)(s0 - S0 + j)];
+                }
             __syncthreads();
             bool live[TPT], diag[TPT], full = cnt == 64, any = false;
 #pragma unroll
@@ -1032,17 +1066,19 @@ __global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ 
                     double v = jl[0][m][j];
 #pragma unroll
                     for (int ww = 1; ww < NW; ++ww) v += jl[ww][m][j];
-                    jrow[(size_t)m * len + (size_t)(s0 - B0 + j)] = v;
+                    jrow[(size_t)m * len + (size_t)(s0 - S0 + j)] = first ? v : prev[m][j] + v;
                 }
             }
         }
+        const size_t pl = sym_part_len((size_t)item.chunk, (size_t)J.npad, (size_t)J.chunk_len, (size_t)TW);
+        double *orow = J.part + sym_part_offset((size_t)item.chunk, (size_t)J.npad, (size_t)J.chunk_len, (size_t)TW);
 #pragma unroll
         for (int p = 0; p < TPT; ++p)
             if (idx[p] < n) {
-                double *o = J.part + (size_t)item.chunk * NCOMP_S * J.npad + idx[p];
 #pragma unroll
-                for (int m = 0; m < NCOMP_S; ++m) o[(size_t)m * J.npad] = a[p].v[m];
+                for (int m = 0; m < NCOMP_S; ++m) orow[(size_t)m * pl + idx[p]] = a[p].v[m];
             }
+        }  // blocks of the strip
     }
 }
 
@@ -1162,35 +1198,39 @@ __global__ __launch_bounds__(256) void combine_kernel(const JobDev *__restrict__
         const int jmode = mode == MODE_PER_JOB ? J.mode : mode;
         if (J.sym) {
             // target side: the chunks from the one holding this particle's
-            // block start; source side: the blocks up to this particle's
-            // own.  Every particle of the item is in the same block (64
-            // divides tw), so the loops and row addresses are uniform per
-            // wave; lanes past n read padding (inside every row) and are
-            // dropped at the end.
+            // block start; source side: the strips up to this particle's
+            // own.  Every particle of the item is in the same block and
+            // strip (64 divides tw, tw divides sw), so the loops and row
+            // addresses are uniform per wave; lanes past n read padding
+            // (inside every row) and are dropped at the end.
             const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);  // the wave
             const int lane = (int)threadIdx.x & 63;
             const int i0 = c.tile * 64, i = i0 + lane;
             const bool live = i < J.n;
-            const size_t tw = (size_t)J.tw;
-            const int blk = i0 / J.tw;
-            const int cfirst = (int)(((size_t)blk * tw) / (size_t)J.chunk_len);
+            const size_t tw = (size_t)J.tw, sw = (size_t)J.sw, L = (size_t)J.chunk_len;
+            const int blk = i0 / J.tw;   // this item's block
+            const int sblk = i0 / J.sw;  // and strip
+            const int cfirst = (int)(((size_t)blk * tw) / L);
             double S[NCOMP_S], T[NCOMP_S];
 #pragma unroll
             for (int m = 0; m < NCOMP_S; ++m) S[m] = T[m] = 0.0;
             for (int ch = cfirst + w; ch < J.nchunk; ch += 4) {
-                const gdouble *p = (const gdouble *)(J.part + (size_t)ch * NCOMP_S * np + i0) + lane;
+                const size_t pl = sym_part_len((size_t)ch, np, L, tw);
+                const gdouble *p =
+                    (const gdouble *)(J.part + sym_part_offset((size_t)ch, np, L, tw) + i0) + lane;
 #pragma unroll
-                for (int m = 0; m < NCOMP_S; ++m) S[m] += p[m * np];
+                for (int m = 0; m < NCOMP_S; ++m) S[m] += p[m * pl];
             }
-            // two rows in flight per step (added in row order)
+            // source side: the rows of the strips up to this item's; two rows
+            // in flight per step (added in row order)
             int b = w;
-            for (; b + 4 <= blk; b += 8) {
-                const size_t len0 = np - (size_t)b * tw, len1 = len0 - 4 * tw;
+            for (; b + 4 <= sblk; b += 8) {
+                const size_t len0 = np - (size_t)b * sw, len1 = len0 - 4 * sw;
                 const gdouble *q0 =
-                    (const gdouble *)(J.jpart + jpart_offset((size_t)b, np, tw) + (i0 - (size_t)b * tw)) +
+                    (const gdouble *)(J.jpart + jpart_offset((size_t)b, np, sw) + (i0 - (size_t)b * sw)) +
                     lane;
-                const gdouble *q1 = (const gdouble *)(J.jpart + jpart_offset((size_t)b + 4, np, tw) +
-                                                      (i0 - ((size_t)b + 4) * tw)) +
+                const gdouble *q1 = (const gdouble *)(J.jpart + jpart_offset((size_t)b + 4, np, sw) +
+                                                      (i0 - ((size_t)b + 4) * sw)) +
                                     lane;
                 double v0[NCOMP_S], v1[NCOMP_S];
 #pragma unroll
@@ -1201,10 +1241,10 @@ __global__ __launch_bounds__(256) void combine_kernel(const JobDev *__restrict__
 #pragma unroll
                 for (int m = 0; m < NCOMP_S; ++m) T[m] = (T[m] + v0[m]) + v1[m];
             }
-            if (b <= blk) {
-                const size_t len = np - (size_t)b * tw;
+            if (b <= sblk) {
+                const size_t len = np - (size_t)b * sw;
                 const gdouble *q =
-                    (const gdouble *)(J.jpart + jpart_offset((size_t)b, np, tw) + (i0 - (size_t)b * tw)) +
+                    (const gdouble *)(J.jpart + jpart_offset((size_t)b, np, sw) + (i0 - (size_t)b * sw)) +
                     lane;
 #pragma unroll
                 for (int m = 0; m < NCOMP_S; ++m) T[m] += q[m * len];

@@ -71,11 +71,14 @@ size_t chunk_for(size_t n, int TB) {
 
 size_t chunk_for_sym(size_t n, int TB, int TPT) {
     if (n == 0) return (size_t)TB;
-    const size_t ntile = (n + TB - 1) / TB;
-    const size_t nblk = (n + (size_t)TB * TPT - 1) / ((size_t)TB * TPT);
-    size_t nchunk = std::min<size_t>(64, std::max<size_t>(16, (16384 + nblk - 1) / nblk));
-    nchunk = std::min(nchunk, ntile);
-    return (size_t)TB * ((ntile + nchunk - 1) / nchunk);
+    const size_t tw = (size_t)TB * TPT, nblk = (n + tw - 1) / tw;
+    const size_t nstrip = std::min(nblk, std::max<size_t>(128, nblk / 8));
+    return tw * ((nblk + nstrip - 1) / nstrip);
+}
+
+size_t strip_for_sym(size_t L, int TB, int TPT) {
+    const size_t tw = (size_t)TB * TPT;
+    return tw * std::max<size_t>(1, L / tw);
 }
 
 static size_t rec_bytes(int precision) {
@@ -113,9 +116,14 @@ static void fill_records(void *dst, int precision, size_t m, const double *pos, 
     }
 }
 
-size_t jpart_count(size_t n, size_t npad, size_t tw) {
-    const size_t nblk = (n + tw - 1) / tw;
-    return jpart_offset(nblk, npad, tw);
+size_t jpart_count(size_t n, size_t npad, size_t sw) {
+    const size_t nstrip = (n + sw - 1) / sw;
+    return jpart_offset(nstrip, npad, sw);
+}
+
+size_t part_count(size_t n, size_t npad, size_t L, size_t tw, bool sym) {
+    const size_t nchunk = (n + L - 1) / L;
+    return sym ? sym_part_offset(nchunk, npad, L, tw) : (size_t)NCOMP * nchunk * npad;
 }
 
 int block_for(int block, int precision) {
@@ -126,11 +134,9 @@ size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym) {
     TB = block_for(TB, precision);
     size_t npad = round_up(std::max<size_t>(n, 1), 256);
     size_t L = sym ? chunk_for_sym(n, TB, TPT) : chunk_for(n, TB);
-    size_t nchunk = (n + L - 1) / L;
-    size_t b = npad * (rec_bytes(precision) + 5 * 6 * sizeof(double) + 2 * sizeof(double) +
-                       nchunk * NCOMP * sizeof(double)) +
-               256;
-    if (sym) b += jpart_count(npad, npad, (size_t)TB * TPT) * sizeof(double);
+    size_t b = npad * (rec_bytes(precision) + 5 * 6 * sizeof(double) + 2 * sizeof(double)) +
+               part_count(n, npad, L, (size_t)TB * TPT, sym) * sizeof(double) + 256;
+    if (sym) b += jpart_count(npad, npad, strip_for_sym(L, TB, TPT)) * sizeof(double);
     return b;
 }
 
@@ -157,6 +163,7 @@ JobDev Job::dev() const {
     d.uniform = uniform ? 1 : 0;
     d.sym = sym ? 1 : 0;
     d.tw = tw;
+    d.sw = sw;
     d.jpart = jpart.as<double>();
     d.dt = par.dt;
     d.nu = par.nu;
@@ -358,7 +365,8 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     j->nchunk = (int)((n + L - 1) / L);
     j->ntile = (int)((n + tb * tpt - 1) / (tb * tpt));
     j->chunk_override = chunk_override;
-    j->part_chunks = j->nchunk;
+    j->tw = tb * tpt;
+    j->sw = (int)strip_for_sym(L, tb, tpt);
     j->uniform = true;
     for (size_t i = 1; i < n; ++i)
         if (std::memcmp(&radius[i], &radius[0], sizeof(double)) != 0) {
@@ -369,10 +377,8 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     const size_t np = j->npad;
     j->rec.reset(np * rec_bytes(par.precision));
     j->soa.reset(5 * 6 * np * sizeof(double));
-    j->part.reset((size_t)j->nchunk * NCOMP * np * sizeof(double));
     j->vol.reset(2 * np * sizeof(double));  // volume | radius
-    j->tw = tb * tpt;
-    alloc_jpart(j.get());
+    alloc_part(j.get());
     j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + sizeof(int) +
                j->jpart.bytes;
     // host-side staging of records, state and constants
@@ -433,7 +439,6 @@ void Engine::append(Job *j, size_t m, const double *pos, const double *vort, con
         std::swap(j->vol.p, vol.p);
         std::swap(j->vol.bytes, vol.bytes);
         j->npad = np2;
-        j->part_chunks = 0;  // partial sums are scratch: reallocated below
     }
     const size_t np = j->npad;
     // new particles' records, state (x0 = xs between steps), volume, radius
@@ -481,23 +486,24 @@ void Engine::append(Job *j, size_t m, const double *pos, const double *vort, con
     j->chunk_len = (int)L;
     j->nchunk = (int)((n2 + L - 1) / L);
     j->ntile = (int)((n2 + tb * tpt - 1) / (tb * tpt));
-    if (j->nchunk > j->part_chunks) {
-        j->part_chunks = std::max(j->nchunk, 16);
-        j->part.reset((size_t)j->part_chunks * NCOMP * np * sizeof(double));
-    }
-    alloc_jpart(j);
+    j->sw = (int)strip_for_sym(L, tb, tpt);
+    alloc_part(j);
     j->bytes = j->rec.bytes + j->soa.bytes + j->part.bytes + j->vol.bytes + sizeof(int) +
                j->jpart.bytes;
     plan_dirty_ = true;
 }
 
-// Source-side slots of a symmetric job, sized for its padded capacity (npad
-// particles), so that particle emission reallocates them only when the
-// stride grows (geometrically), not on every append.  Scratch like the
-// partial sums: rewritten by every pass.
-void Engine::alloc_jpart(Job *j) {
+// Partial-sum buffers of a job: the target-side rows (per source chunk) and,
+// on the symmetric pass, the source-side slot rows (per strip).  Scratch,
+// rewritten by every pass; grown only (particle emission reallocates them when
+// the chunk plan needs more than they hold, not on every append).
+void Engine::alloc_part(Job *j) {
+    const size_t np = j->npad;
+    const size_t pneed =
+        part_count(j->n, np, (size_t)j->chunk_len, (size_t)j->tw, j->sym) * sizeof(double);
+    if (pneed > j->part.bytes) j->part.reset(pneed);
     if (!j->sym) return;
-    const size_t need = jpart_count(j->npad, j->npad, (size_t)j->tw) * sizeof(double);
+    const size_t need = jpart_count(np, np, (size_t)j->sw) * sizeof(double);
     if (need > j->jpart.bytes) j->jpart.reset(need);
 }
 
@@ -527,14 +533,19 @@ static void job_items(const Job &j, int k, std::vector<Item> &dst, std::vector<C
         for (int c = 0; c < j.nchunk; ++c)
             for (int t = 0; t < j.ntile; ++t) dst.push_back(Item{k, t, c, j.chunk_len});
     } else {
-        // block t pairs with the sources from its own start on: chunks
-        // from the one holding that start; cost = sources visited
-        for (int t = 0; t < j.ntile; ++t) {
-            const size_t b0 = (size_t)t * j.tw;
-            for (int c = (int)(b0 / j.chunk_len); c < j.nchunk; ++c) {
-                const size_t lo = std::max(b0, (size_t)c * j.chunk_len);
-                const size_t hi = std::min(j.n, (size_t)(c + 1) * j.chunk_len);
-                dst.push_back(Item{k, t, c, (int)(hi - lo)});
+        // strip s (blocks of tw targets from s sw on) pairs with the
+        // sources from its start on: chunks from the one holding that start;
+        // cost = sources its blocks visit
+        const size_t nstrip = (j.n + j.sw - 1) / j.sw;
+        for (size_t t = 0; t < nstrip; ++t) {
+            const size_t s0 = t * j.sw, s1 = std::min(j.n, s0 + j.sw);
+            for (int c = (int)(s0 / j.chunk_len); c < j.nchunk; ++c) {
+                const size_t clo = (size_t)c * j.chunk_len;
+                const size_t chi = std::min(j.n, clo + j.chunk_len);
+                size_t cost = 0;
+                for (size_t b0 = s0; b0 < s1; b0 += j.tw)
+                    if (std::max(b0, clo) < chi) cost += chi - std::max(b0, clo);
+                dst.push_back(Item{k, (int)t, c, (int)std::min<size_t>(cost, INT32_MAX)});
             }
         }
     }
@@ -560,8 +571,9 @@ static void order_combine_items(std::vector<CItem> &ci, const std::vector<const 
         const Job &j = *tab[(size_t)c.job];
         if (!j.sym) return (size_t)NCOMP * 256 * (size_t)j.nchunk;
         const size_t b0 = (size_t)c.tile * 64 / (size_t)j.tw;
+        const size_t s0 = (size_t)c.tile * 64 / (size_t)j.sw;
         const size_t cf = b0 * (size_t)j.tw / (size_t)j.chunk_len;
-        return (size_t)NCOMP_S * 64 * (b0 + 1 + (size_t)j.nchunk - cf);
+        return (size_t)NCOMP_S * 64 * (s0 + 1 + (size_t)j.nchunk - cf);
     };
     std::vector<std::pair<size_t, CItem>> kc;
     kc.reserve(ci.size());

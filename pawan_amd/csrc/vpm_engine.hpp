@@ -61,11 +61,17 @@ struct JobParams {
 // so a job's sums are formed the same way alone or in any batch.
 size_t chunk_for(size_t n, int TB);
 // Source chunk length for a job on the symmetric pass (blocks of TB x TPT
-// targets): enough chunks that the upper-triangle work items number at least
-// ~16,384 / 2 (8 per workgroup of a full persistent grid), so that the last
-// round of items is a small part of the launch; 16 to 64 chunks, a multiple
-// of TB.  Depends on (n, TB, TPT) only.
+// targets): a whole number of blocks, so that the chunks and the strips
+// (strip_for_sym) are the same 128 (or, past 1,024 blocks, blocks / 8)
+// stretches of particles.  The upper triangle of (strip, chunk) work items
+// then numbers ~8,256 or more (16 per workgroup of a full persistent grid),
+// and a particle's partial sums take about as many target-side rows as
+// source-side ones (~128 in all).  Depends on (n, TB, TPT) only.
 size_t chunk_for_sym(size_t n, int TB, int TPT);
+// Strip width of a symmetric job with source chunks of L (a whole number of
+// blocks of TB x TPT targets, about L): a work item is a strip's blocks
+// against one chunk, and the strip's source-side sums fill one slot row.
+size_t strip_for_sym(size_t L, int TB, int TPT);
 
 // Bytes a resident job occupies on the device (TB x TPT targets per block;
 // sym: the job uses the symmetric pass and its source-side slots).
@@ -75,9 +81,12 @@ size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym);
 // thread (half the source-side slot rows of 256), so an engine asked for 512
 // runs its binary64 jobs at 256.
 int block_for(int block, int precision);
-// Binary64 sums in a symmetric job's source-side slots (block rows of
-// NCOMP_S x (npad - b tw) for every block b).
-size_t jpart_count(size_t n, size_t npad, size_t tw);
+// Binary64 sums in a symmetric job's source-side slots (strip rows of
+// NCOMP_S x (npad - s sw) for every strip s).
+size_t jpart_count(size_t n, size_t npad, size_t sw);
+// Binary64 sums in a job's target-side partial rows (symmetric: triangular,
+// sym_part_offset; one-sided: nchunk x NCOMP x npad).
+size_t part_count(size_t n, size_t npad, size_t L, size_t tw, bool sym);
 
 struct Diagnostics {
     double omega[3], impulse[3], angular[3];
@@ -99,9 +108,9 @@ struct Job {
     DevBuf jpart;               // source-side slots of the symmetric pass
     bool sym = false;           // binary32 job on the symmetric pass
     int tw = 0;                 // targets per block of the pass
+    int sw = 0;                 // targets per strip (symmetric pass)
     size_t bytes = 0;
     size_t chunk_override = 0;  // fixed chunk length (0: policy)
-    int part_chunks = 0;        // chunks the partial-sum buffer holds
     JobDev dev() const;
     // factor applied to positions in the records (1/sigma for a uniform core)
     double rec_scale() const { return uniform ? 1.0 / sigma : 1.0; }
@@ -225,7 +234,7 @@ class Engine {
     int tb_of(int precision) const { return precision == PREC_FP32C ? tb32_ : tb_; }
     int tpt32_;  // targets per thread of binary32 jobs (tpt_: binary64 jobs)
     int tpt_of(int precision) const { return precision == PREC_FP32C ? tpt32_ : tpt_; }
-    void alloc_jpart(Job *j);
+    void alloc_part(Job *j);
     void (*fn64_)(const JobDev *, const Item *, const int *, int *) = nullptr;
     void (*fn32_)(const JobDev *, const Item *, const int *, int *) = nullptr;
     hipStream_t stream_ = nullptr;

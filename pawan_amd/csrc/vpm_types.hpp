@@ -85,7 +85,8 @@ struct JobDev {
     double *k1;     // [6][npad] RK4 k1
     double *k23;    // [6][npad] RK4 k2 (+ k3)
     double *rate;   // [6][npad] last rates (u, da/dt)
-    double *part;   // [nchunk][NCOMP][npad] per-chunk partial sums
+    double *part;   // per-chunk target-side partial sums: one-sided [nchunk][NCOMP][npad];
+                    // symmetric: chunk c's row [NCOMP_S][sym_part_len(c)] at sym_part_offset(c)
     const double *vol;  // [npad] volumes
     int *flag;      // set to 1 when a rate or state is non-finite
     int n, npad, nchunk, chunk_len;
@@ -94,8 +95,10 @@ struct JobDev {
     int sym;        // binary32 symmetric pass: part holds NCOMP_S target-side
                     // components, jpart the source-side ones
     int tw;         // targets per block of the symmetric pass
-    double *jpart;  // source-side slots: block b holds [NCOMP_S][npad - b tw]
-                    // sums for particles b tw .. npad-1, at jpart_offset(b)
+    int sw;         // targets per strip (a whole number of blocks): the work of a
+                    // strip's blocks against one source chunk is one work item
+    double *jpart;  // source-side slots: strip s holds [NCOMP_S][npad - s sw]
+                    // sums for particles s sw .. npad-1, at jpart_offset(s)
     double inv_s2;  // non-uniform core: unused; uniform: 1 (positions are prescaled)
     double rec_scale;  // records hold positions x rec_scale (1/sigma for a uniform core)
     double su, sq, sf, sd;  // epilogue scales (see combine_kernel)
@@ -103,10 +106,26 @@ struct JobDev {
     double uinf[3];
 };
 
-// Start of block b's source-side slot row (rows shrink by tw per block:
-// block b only reaches particles from b tw on).
-__host__ __device__ inline size_t jpart_offset(size_t b, size_t npad, size_t tw) {
-    return (size_t)NCOMP_S * (b * npad - tw * (b * (b - 1) / 2));
+// Start of strip s's source-side slot row (rows shrink by sw per strip: the
+// strip's blocks only reach particles from s sw on).
+__host__ __device__ inline size_t jpart_offset(size_t s, size_t npad, size_t sw) {
+    return (size_t)NCOMP_S * (s * npad - sw * (s * (s - 1) / 2));
+}
+
+// Target-side rows of a symmetric job.  Row c (source chunk c, L sources)
+// holds the sums of the targets of the blocks that start before the chunk
+// ends, [0, sym_part_len(c)) -- when tw divides L; otherwise every row holds
+// all npad targets.
+__host__ __device__ inline size_t sym_part_len(size_t c, size_t npad, size_t L, size_t tw) {
+    if (L % tw != 0) return npad;
+    const size_t e = (c + 1) * L;
+    return e < npad ? e : npad;
+}
+__host__ __device__ inline size_t sym_part_offset(size_t c, size_t npad, size_t L, size_t tw) {
+    if (L % tw != 0) return (size_t)NCOMP_S * c * npad;
+    const size_t cm = npad / L;  // rows c' < cm hold (c' + 1) L targets, later ones npad
+    if (c <= cm) return (size_t)NCOMP_S * L * (c * (c + 1) / 2);
+    return (size_t)NCOMP_S * (L * (cm * (cm + 1) / 2) + (c - cm) * npad);
 }
 
 struct Item {

@@ -48,21 +48,27 @@ def test_host_only_entry_points():
             L = lib.pw_chunk_for(n, b)
             assert L % b == 0 and (n + L - 1) // L <= 16
     assert lib.pw_job_footprint(262144, 0) > 262144 * 64
-    # binary32 records carry binary64 strengths too: 80 B instead of 64 B
-    # binary32: 16 more bytes per record, plus the symmetric pass's source-side
-    # slots, NCOMP_S = 10 binary64 sums per (particle, block of targets) for
-    # the blocks at or before the particle's own; blocks of 256 targets
-    # (256 x 1) up to 524,288 particles, 512 (512 x 1) beyond.  The
-    # symmetric pass also takes 16-64 source chunks
-    # (enough that its triangle of work items is ~8,192 long) where the
-    # one-sided pass takes 16: 13 binary64 partial sums per chunk and particle.
-    for n, tw, nch in ((262144, 256, 16), (524288, 256, 16), (786432, 512, 16), (65536, 256, 64),
-                       (131072, 256, 32)):
+    # binary32 records carry binary64 strengths too: 80 B instead of 64 B.
+    # (Contexts under 8,192 particles take 64-thread blocks.)
+    # The symmetric pass keeps NCOMP_S = 10 binary64 sums per particle and
+    # partial row: blocks of tw = 256 targets (256 x 1) up to 524,288
+    # particles, 512 (512 x 1) beyond; source chunks and strips of L =
+    # tw x ceil(nblk / nstrip) particles, nstrip = min(nblk, max(128, nblk / 8));
+    # target-side row c holds the targets below (c + 1) L, source-side row s
+    # the particles from s L on.  The one-sided binary64 pass keeps 13 sums
+    # per particle and chunk, 16 chunks.
+    for n, tw in ((262144, 256), (524288, 256), (786432, 512), (65536, 256), (131072, 256),
+                  (4096, 64)):
         npad = n
         nblk = npad // tw
-        slots = 10 * (nblk * npad - tw * nblk * (nblk - 1) // 2)
-        assert (lib.pw_job_footprint(n, 1) - lib.pw_job_footprint(n, 0)
-                == 16 * n + 8 * slots + (nch - 16) * 13 * 8 * npad), n
+        nstrip = min(nblk, max(128, nblk // 8))
+        L = tw * -(-nblk // nstrip)
+        tri = sum(min((c + 1) * L, npad) for c in range(-(-n // L)))
+        rows = sum(npad - s * L for s in range(-(-npad // L)))
+        want = 16 * n + 8 * 10 * (tri + rows) - 8 * 13 * 16 * npad
+        assert lib.pw_job_footprint(n, 1) - lib.pw_job_footprint(n, 0) == want, n
+    # the slots of a 262,144-particle binary32 job: under 3 GB (1.35 + 1.35)
+    assert lib.pw_job_footprint(262144, 1) < 3.0e9
 
 
 def test_null_arguments_rejected():
