@@ -101,7 +101,8 @@ def run_path(pa, w, precision, steps, warmup, variant=None):
         "pair_kernel_ms_mean": tm["pair_ms"] / launches,
         "pairs_per_launch": n * (n - 1.0),
         "pairs_per_s": pairs / wall,
-        "drift": dict(_drift(d0, d1), steps=warmup + steps, dt=dt),
+        "drift": dict(_drift(d0, d1, float(np.linalg.norm(w["vort"], axis=1).sum())),
+                      steps=warmup + steps, dt=dt),
     }
     return S, out
 
@@ -336,14 +337,17 @@ def copy_calibration(mib):
 
 
 # ------------------------------------------------------------------ workloads
-def _drift(d0, d1):
+def _drift(d0, d1, strength=None):
     """Change of total vorticity and linear impulse (interaction.cpp:240-259):
-    |change| and |change| / |initial|; kinetic energy (:351-366): relative change."""
+    |change|, and |change| relative to |initial| (total vorticity: relative to
+    the summed strength magnitudes `strength` when given, since a wake's total
+    vorticity can start at zero); kinetic energy (:351-366): relative change."""
     out = {}
     for k in ("total_vorticity", "linear_impulse"):
         a, b = np.asarray(d0[k]), np.asarray(d1[k])
         out[k] = float(np.linalg.norm(b - a))
-        out[k + "_rel"] = float(np.linalg.norm(b - a) / max(np.linalg.norm(a), 1e-300))
+        ref = strength if (k == "total_vorticity" and strength) else np.linalg.norm(a)
+        out[k + "_rel"] = float(np.linalg.norm(b - a) / max(ref, 1e-300))
     out["kinetic_energy_rel"] = float((d1["kinetic_energy"] - d0["kinetic_energy"]) /
                                       d0["kinetic_energy"])
     return out

@@ -276,12 +276,15 @@ size_t pw_chunk_for(size_t n, int block);
  * every other resident job, rebatched every pass as jobs join and finish, and
  * retired with their final state kept until fetched.
  *
- * A pass advances every resident job by 1, 2 or 4 RK4 stages (never past
- * the end of its current step): a job whose stage costs at most a quarter
- * (half) of the largest resident job's stage, or of a floor of 2e9 pair
- * interactions, runs 4 (2) stages, the others 1, so that small jobs take
- * whole steps while a large job takes one stage per pass, and no pass holds a
- * whole step of the largest job.  Each stage is the same computation
+ * A pass advances resident jobs by 1, 2 or 4 RK4 stages (never past the end
+ * of a job's current step).  The reference cost is the largest resident
+ * job's stage, or 2e9 pair interactions if that is more.  A job whose stage
+ * costs at most a quarter of it ("small") runs 4 stages in every pass; the
+ * others ("large") run 2 if their stage costs at most half of it, else 1,
+ * and take turns: a pass takes them in id order from a rotating cursor while
+ * their work stays within one reference stage (at least one).  So small jobs
+ * take whole steps per pass, no pass holds more than about one large stage
+ * besides them, and every job advances.  Each stage is the same computation
  * whichever pass it falls in, so a job's result does not depend on the mix.
  * Jobs that arrive during a pass are uploaded beside it (a second stream)
  * and join the next one; the non-finite flags of all jobs come back in one
@@ -312,10 +315,10 @@ double pw_server_now_ms(const pw_server *s);
 int pw_server_submit(pw_server *s, const pw_job_spec *spec, size_t nsteps, double arrival_ms,
                      int64_t *id);
 int pw_server_cancel(pw_server *s, int64_t id);
-/* one pass of the request loop: queue a pass over every resident job (1, 2
- * or 4 RK4 stages each), admit queued jobs that fit the budget while it
- * runs, then retire finished and failed jobs.  *running receives the number
- * of resident jobs after the pass. */
+/* one pass of the request loop: queue a pass (the small jobs and a turn of
+ * the large ones, 1, 2 or 4 RK4 stages each), admit queued jobs that fit the
+ * budget while it runs, then retire finished and failed jobs.  *running
+ * receives the number of resident jobs after the pass. */
 int pw_server_step(pw_server *s, int64_t *running);
 /* 0 queued, 1 running, 2 done, 3 failed (non-finite), 4 cancelled, -1 unknown */
 int pw_server_job_state(pw_server *s, int64_t id, int64_t *steps_done);

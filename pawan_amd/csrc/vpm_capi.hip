@@ -698,6 +698,7 @@ struct pw_server {
     std::map<int64_t, SJob> jobs;
     std::deque<int64_t> queue;     // arrival order
     std::vector<int64_t> resident;  // admission order
+    int64_t cursor = 0;             // large jobs: the next pass starts at this id
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     std::vector<double> step_ms, job_ms, job_step_ms;
     pw_server_stats st{};
@@ -823,13 +824,42 @@ int pw_server_step(pw_server *s, int64_t *running) {
         const double t_begin = s->now();
         // an idle server admits first, so that arrivals start in this pass
         if (s->resident.empty()) s->admit(false, &s->resident);
-        // the pass: every resident job's stages for it
+        // the pass: every small job's stages (a whole step at most), and the
+        // large jobs' (1 or 2 stages each) taken in turn from a rotating
+        // cursor while their work stays within one largest stage, at least one
         std::vector<pw::PassJob> pj;
+        std::vector<int64_t> pid;
         double ref = kPassFloor;
         for (int64_t id : s->resident) ref = std::max(ref, stage_cost(s->jobs[id]));
+        std::vector<int64_t> large;
         for (int64_t id : s->resident) {
             SJob &j = s->jobs[id];
-            pj.push_back(pw::PassJob{j.job, j.stage, stages_this_pass(j, ref)});
+            const int k = stages_this_pass(j, ref);
+            if (4 * stage_cost(j) <= ref) {
+                pj.push_back(pw::PassJob{j.job, j.stage, k});
+                pid.push_back(id);
+            } else {
+                large.push_back(id);
+            }
+        }
+        if (!large.empty()) {
+            // in id order, from the first large job at or after the cursor
+            std::sort(large.begin(), large.end());
+            size_t first = 0;
+            while (first < large.size() && large[first] < s->cursor) ++first;
+            if (first == large.size()) first = 0;
+            double work = 0.0;
+            for (size_t t = 0; t < large.size(); ++t) {
+                const int64_t id = large[(first + t) % large.size()];
+                SJob &j = s->jobs[id];
+                const int k = stages_this_pass(j, ref);
+                const double wk = k * stage_cost(j);
+                if (t > 0 && work + wk > ref) break;
+                work += wk;
+                pj.push_back(pw::PassJob{j.job, j.stage, k});
+                pid.push_back(id);
+                s->cursor = id + 1;
+            }
         }
         std::vector<int64_t> joining;
         std::exception_ptr admit_err;
@@ -848,10 +878,18 @@ int pw_server_step(pw_server *s, int64_t *running) {
             s->st.steps++;
             s->st.pairs += s->eng->last_pairs;
             s->st.busy_ms += s->eng->last_total_ms;
-            // advance, then retire finished and failed jobs
+            // advance the jobs of the pass, then retire finished and failed
+            // ones; the others keep their place (admission order)
+            std::map<int64_t, size_t> in_pass;
+            for (size_t k = 0; k < pid.size(); ++k) in_pass[pid[k]] = k;
             std::vector<int64_t> keep;
-            for (size_t k = 0; k < pj.size(); ++k) {
-                const int64_t id = s->resident[k];
+            for (const int64_t id : s->resident) {
+                auto ip = in_pass.find(id);
+                if (ip == in_pass.end()) {
+                    keep.push_back(id);
+                    continue;
+                }
+                const size_t k = ip->second;
                 SJob &j = s->jobs[id];
                 j.stage = (j.stage + pj[k].stages) % 4;
                 j.stages_left -= (size_t)pj[k].stages;

@@ -185,3 +185,40 @@ def test_server_passes_by_stage():
           f"job step p50 {st['job_step_p50_ms']:.2f} ms", flush=True)
     assert st["hbm_in_use"] == 0 and st["cancelled"] == 1
     srv.close()
+
+
+def test_server_large_jobs_take_turns():
+    """Two large jobs (65,536-particle rotor wakes, one stage each the whole
+    reference cost) take turns pass by pass while a small ring takes a whole
+    step every pass: the ring is done after 2 passes, each wake after 4 of
+    its own stages (passes 7 and 8), and all three results equal their
+    results alone."""
+    big = gen.rotor_wake(256)
+    small = gen.vortex_ring(128, 16, seed=4)
+    jobs = [(big, 1e-4, 1e-5, 1), (dict(big, pos=big["pos"] + np.array([0.0, 0.0, 3.0])), 1e-4,
+                                   1e-5, 1), (small, 1e-3, 0.005, 2)]
+    print("case server: two 65,536-particle wakes taking turns, one 2,048-particle ring", flush=True)
+    alone = []
+    for w, nu, dt, steps in jobs:
+        srv = pa.Server(64 << 30)
+        jid = srv.submit(w, nu, dt, steps)
+        while srv.step() > 0:
+            pass
+        alone.append(srv.fetch(jid))
+        srv.close()
+    srv = pa.Server(64 << 30)
+    ids = [srv.submit(w, nu, dt, steps) for w, nu, dt, steps in jobs]
+    done_at = {}
+    for p in range(1, 9):
+        running = srv.step()
+        for k, i in enumerate(ids):
+            if k not in done_at and srv.state(i)[0] == pa.Server.DONE:
+                done_at[k] = p
+        if p == 2:
+            assert srv.state(ids[0])[1] == 0 and srv.state(ids[1])[1] == 0
+    assert running == 0
+    assert done_at == {2: 2, 0: 7, 1: 8}, done_at
+    for k, i in enumerate(ids):
+        pos, vort = srv.fetch(i)
+        assert np.array_equal(pos, alone[k][0]) and np.array_equal(vort, alone[k][1]), k
+    srv.close()
