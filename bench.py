@@ -48,7 +48,7 @@ FP32_VECTOR_PEAK = 157.3e12   # MI355X FP32 vector FLOP/s (published, packed)
 # from rocprofv3 FETCH_SIZE + WRITE_SIZE passes in KiB per dispatch, scaled by
 # what the same counters read for a cold 2 GiB copy of known size (fetch x2,
 # write x1); profiles/pair_kernels_rotor_wake_262144.md
-PAIR64_TRAFFIC_262144 = (2 * 210_931 + 426_256) * 1024.0
+PAIR64_TRAFFIC_262144 = (2 * 211_721 + 426_256) * 1024.0
 
 
 def log(*a):
