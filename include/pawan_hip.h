@@ -327,8 +327,8 @@ int pw_server_fetch(pw_server *s, int64_t id, double *pos, double *vort);
 int pw_server_get_stats(pw_server *s, pw_server_stats *st);
 /* device bytes one job of n particles occupies while resident, for the
  * default variant of a context of n particles (a binary32 job includes the
- * symmetric pass's source-side slots, 40 n^2 / (block x targets per thread)
- * bytes) */
+ * symmetric pass's slot rows, 80 bytes per particle per strip with
+ * max(128, blocks / 8) strips: 2.8 GB at n = 262,144, 21.9 GB at 1,048,576) */
 size_t pw_job_footprint(size_t n, int precision);
 
 /* -------------------------------------------------------------- autotuner

@@ -47,10 +47,11 @@ int guarded(std::string *err, F &&f) {
 }
 
 // One target per thread is the faster binary32 symmetric pass (59.9 against
-// 68.2 ms at N = 262,144; 244 against 270 ms at 524,288), but its source-side
-// slots take 40 N^2 / (block x targets per thread) bytes: past N = 524,288
-// (43 GB at 256 threads) a binary32 symmetric HIGHORDER job takes 512 threads x
-// 1 target, which halves them as 256 x 2 does with the faster one-target pass
+// 68.2 ms at N = 262,144; 244 against 270 ms at 524,288), but its slots take
+// 80 bytes per particle per strip, and past 128 strips the strip count is
+// N / (8 x block x targets per thread): past N = 524,288 (43 GB at 256 x 1 and
+// N = 1,048,576) a binary32 symmetric HIGHORDER job takes 512 threads x 1
+// target, which halves them as 256 x 2 does with the faster one-target pass
 // (973 against 1,052 ms per RHS at N = 1,048,576).  From 131,072 to 524,288
 // 512 x 1 and 256 x 1 are within 1% of each other (the autotuner picks).
 // An explicit two targets per thread, or a chunk length that 512 does not
