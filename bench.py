@@ -82,7 +82,8 @@ def run_path(pa, w, precision, steps, warmup, variant=None):
     S = pa.Interaction(pa.Wake.from_dict(w), nu=w["nu"], precision=precision,
                        uinf=w.get("uinf", (0, 0, 0)), block=v.get("block", 0),
                        targets_per_thread=v.get("targets_per_thread", 0),
-                       unroll=v.get("unroll", 0), scheme=v.get("scheme", 0))
+                       unroll=v.get("unroll", 0), scheme=v.get("scheme", 0),
+                       chunks=v.get("chunks", 0))
     dt = STEP_DT
     S.solve()  # builds the device job
     d0 = S.diagnose()  # before the warm-up steps (outside the timed region)
@@ -687,9 +688,12 @@ def main():
                     tuned["fp64_symmetric"] = min(sym, key=lambda e: e["pair_ms"])
             tuned[name] = best
             log(f"autotune {name}: best {best} of {len(allv)} variants")
-            for e in sorted(allv, key=lambda e: e["pair_ms"])[:8]:
-                log(f"    {e['block']:4d} x{e['targets_per_thread']} unroll {e['unroll']} "
-                    f"scheme {e['scheme']}: {e['pair_ms']:.2f} ms")
+            for sc in (pa.SCHEME_ONESIDED, pa.SCHEME_SYMMETRIC):
+                for e in sorted((e for e in allv if e["scheme"] == sc),
+                                key=lambda e: e["pair_ms"])[:5]:
+                    log(f"    {e['block']:4d} x{e['targets_per_thread']} unroll {e['unroll']} "
+                        f"scheme {e['scheme']} chunks {e['chunks'] or 'policy'}: "
+                        f"{e['pair_ms']:.2f} ms")
     log(f"workload rotor wake N={n}; FP64 warmup {args.warmup} steps {args.steps}")
     S64, r64 = run_path(pa, w, pa.FP64, args.steps, args.warmup, tuned.get("fp64"))
     log(f"fp64: {r64['pairs_per_s']:.4e} pairs/s, pair kernel {r64['pair_kernel_ms_mean']:.2f} ms")

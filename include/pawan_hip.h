@@ -79,6 +79,8 @@ typedef struct pw_config {
     int targets_per_thread; /* 1 or 2; 0 = tuned or default                      */
     int unroll;       /* source-loop unroll 1/2/4; 0 = tuned or default          */
     int scheme;       /* PW_SCHEME_*: how the pair pass visits pairs             */
+    int chunks;       /* source chunks per job when chunk is 0 (0 = policy or the
+                         tuning cache's); symmetric pass: also the strip count */
 } pw_config;
 
 /* A compiled kernel variant of the pair pass. */
@@ -89,6 +91,8 @@ typedef struct pw_variant {
     int targets_per_thread; /* 1 or 2                                             */
     int unroll;             /* 1, 2 or 4; applies to the precision being tuned    */
     int scheme;             /* PW_SCHEME_* (0: default)                            */
+    int chunks;             /* source chunks per job (0: the chunk policy, at most
+                               16 one-sided, max(128, blocks / 8) symmetric)    */
 } pw_variant;
 
 typedef struct pw_ctx pw_ctx;
@@ -333,9 +337,13 @@ size_t pw_job_footprint(size_t n, int precision);
 
 /* -------------------------------------------------------------- autotuner
  * Times every compiled variant (block x targets per thread x unroll, and
- * both pair schemes) on the given particles (reps solves each, HIP events)
- * and returns them in `all` (up to max_all entries) and the fastest in *best.  One-sided variants give bitwise identical results; symmetric
- * ones group their sums by block and agree to binary32 rounding. */
+ * both pair schemes) on the given particles at the chunk policy, then the
+ * fastest variant of each scheme at 1/2, 3/4, 3/2 and 2x the policy's source
+ * chunk count (reps solves each, HIP events), and returns them in `all` (up
+ * to max_all entries; variant.chunks 0 = the policy) and the fastest in
+ * *best.  One-sided variants give bitwise identical results at one chunk
+ * count; symmetric ones group their sums by block and strip and agree to
+ * binary32 rounding. */
 typedef struct pw_tune_entry {
     pw_variant variant;
     double pair_ms;      /* mean device time per solve: pair pass and
@@ -346,9 +354,12 @@ typedef struct pw_tune_entry {
 int pw_autotune(int device, int kernel, int precision, size_t n, const double *pos,
                 const double *vort, const double *radius, const double *volume, int reps,
                 pw_tune_entry *all, size_t max_all, size_t *n_all, pw_tune_entry *best);
-/* A tuning cache file: one line per (kernel, precision, log2 N bucket).
- * pw_create consults the file named by the PW_TUNE_CACHE environment variable
- * when block, targets_per_thread and unroll are all 0. */
+/* A tuning cache file: one line per (kernel, precision, log2 N bucket):
+ * "kernel precision log2N block targets_per_thread unroll scheme ms chunks"
+ * (a line without the chunks field reads as chunks 0).  pw_create consults the
+ * file named by the PW_TUNE_CACHE environment variable when block,
+ * targets_per_thread, unroll and scheme are all 0; its chunk count applies
+ * when the config's chunk and chunks are 0. */
 int pw_tune_save(const char *path, int kernel, int precision, size_t n, const pw_tune_entry *e);
 int pw_tune_lookup(const char *path, int kernel, int precision, size_t n, pw_variant *out);
 

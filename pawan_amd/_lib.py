@@ -24,12 +24,12 @@ class PwConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("kernel", C.c_int), ("precision", C.c_int),
                 ("nu", C.c_double), ("uinf", C.c_double * 3), ("block", C.c_int),
                 ("chunk", C.c_int), ("targets_per_thread", C.c_int), ("unroll", C.c_int),
-                ("scheme", C.c_int)]
+                ("scheme", C.c_int), ("chunks", C.c_int)]
 
 
 class PwVariant(C.Structure):
     _fields_ = [("block", C.c_int), ("targets_per_thread", C.c_int), ("unroll", C.c_int),
-                ("scheme", C.c_int)]
+                ("scheme", C.c_int), ("chunks", C.c_int)]
 
 
 class PwTuneEntry(C.Structure):

@@ -98,9 +98,11 @@ struct pw_ctx {
 
     size_t total() const { return (size_t)woff.back(); }
 
-    // the variant for a context of n particles (default: the current total)
-    pw::Variant choose_variant() const { return choose_variant(total()); }
-    pw::Variant choose_variant(size_t n) const {
+    // the variant for a context of n particles (default: the current total);
+    // *chunks receives the source chunk count its job takes (0: the policy)
+    pw::Variant choose_variant(int *chunks = nullptr) const { return choose_variant(total(), chunks); }
+    pw::Variant choose_variant(size_t n, int *chunks = nullptr) const {
+        if (chunks) *chunks = cfg.chunks;
         pw::Variant v;
         v.block = cfg.block ? cfg.block
                             : default_block(n, cfg.precision, cfg.kernel, cfg.scheme,
@@ -116,6 +118,7 @@ struct pw_ctx {
             if (pw_tune_lookup(cache, cfg.kernel, cfg.precision, n, &t) == PW_OK) {
                 v.block = t.block;
                 v.tpt = t.targets_per_thread;
+                if (chunks && !cfg.chunks) *chunks = t.chunks;
                 if (cfg.precision == PW_FP64) {
                     if (t.unroll) v.unroll64 = t.unroll;
                     v.sym64 = t.scheme == PW_SCHEME_SYMMETRIC;
@@ -140,7 +143,8 @@ struct pw_ctx {
 
     void rebuild() {
         if (total() == 0) throw Error(PW_EINVAL, "no particles: add a wake first");
-        pw::Variant v = choose_variant();
+        int chunks = 0;
+        pw::Variant v = choose_variant(&chunks);
         if (writer) writer->quiesce();  // queued frames read the current device state
         if (!engine_runs(v)) {
             job = nullptr;
@@ -156,6 +160,7 @@ struct pw_ctx {
         par.dt = dt;
         std::memcpy(par.uinf, cfg.uinf, sizeof par.uinf);
         par.precision = cfg.precision;
+        par.chunks = chunks;
         job = eng->add_job(1, total(), pos.data(), vort.data(), rad.data(), vol.data(), par,
                            (size_t)std::max(0, cfg.chunk));
         dirty = false;
@@ -209,6 +214,7 @@ int pw_create(pw_ctx **out, const pw_config *cfg) {
                         "block 512 is the binary32 symmetric HIGHORDER pass, one target per thread");
         if (c.chunk < 0 || (c.chunk && c.block && c.chunk % c.block))
             throw Error(PW_EINVAL, "chunk must be a non-negative multiple of block");
+        if (c.chunks < 0) throw Error(PW_EINVAL, "chunks must be non-negative");
         if (c.targets_per_thread < 0 || c.targets_per_thread > 2)
             throw Error(PW_EINVAL, "targets_per_thread must be 0, 1 or 2");
         if (c.unroll != 0 && c.unroll != 1 && c.unroll != 2 && c.unroll != 4)
@@ -447,11 +453,13 @@ int pw_append_particles(pw_ctx *ctx, size_t wake, size_t n, const double *pos, c
         if (n == 0) return;
         if (!pos || !vort || !radius || !volume) throw Error(PW_EINVAL, "null array");
         const size_t last = ctx->wake_n.size() - 1;
-        // in place only while the grown context keeps its kernel variant (a
-        // binary32 context past 524,288 particles moves to 512 threads, which
-        // halves its source-side slots); otherwise the next call rebuilds it
-        if (ctx->job && !ctx->dirty && wake == last &&
-            ctx->engine_runs(ctx->choose_variant(ctx->total() + n))) {
+        // in place only while the grown context keeps its kernel variant and
+        // chunk count (a binary32 context past 524,288 particles moves to 512
+        // threads, which halves its slots); otherwise the next call rebuilds it
+        int chunks = 0;
+        const pw::Variant grown = ctx->choose_variant(ctx->total() + n, &chunks);
+        if (ctx->job && !ctx->dirty && wake == last && ctx->engine_runs(grown) &&
+            chunks == ctx->job->par.chunks) {
             // the wake's particles end the device arrays: append in place
             ctx->eng->append(ctx->job, n, pos, vort, radius, volume);
             ctx->pos.insert(ctx->pos.end(), pos, pos + 3 * n);
@@ -485,6 +493,7 @@ int pw_get_variant(pw_ctx *ctx, pw_variant *out) {
         const bool sym = fp32 ? v.sym32 != 0 : v.sym64 != 0;
         out->unroll = sym ? 0 : (fp32 ? v.unroll32 : v.unroll64);
         out->scheme = sym ? PW_SCHEME_SYMMETRIC : PW_SCHEME_ONESIDED;
+        out->chunks = ctx->job->nchunk;
     });
 }
 
@@ -515,6 +524,7 @@ int pw_last_timing(const pw_ctx *ctx, double *total_ms, double *pair_ms, double 
 struct pw_batch {
     std::string err;
     std::unique_ptr<Engine> eng;
+    int chunks = 0;  // source chunks per job (pw_variant::chunks)
     int64_t next_id = 1;
     std::map<int64_t, int> status;
 };
@@ -545,6 +555,7 @@ static pw::Variant variant_of(const pw_variant *v, int precision_hint, int kerne
     if (!v) return r;
     if (v->scheme < PW_SCHEME_DEFAULT || v->scheme > PW_SCHEME_SYMMETRIC)
         throw Error(PW_EINVAL, "unknown pair scheme");
+    if (v->chunks < 0) throw Error(PW_EINVAL, "chunks must be non-negative");
     r.block = v->block ? v->block : 256;
     if (v->targets_per_thread) r.tpt = r.tpt32 = v->targets_per_thread;
     r.sym32 = v->scheme != PW_SCHEME_ONESIDED;
@@ -566,6 +577,7 @@ int pw_batch_create(pw_batch **out, int device, int kernel, const pw_variant *va
     return guarded(nullptr, [&] {
         auto b = std::make_unique<pw_batch>();
         b->eng.reset(new Engine(device, kernel, variant_of(variant, -1, kernel)));
+        b->chunks = variant ? variant->chunks : 0;
         *out = b.release();
     });
 }
@@ -583,6 +595,7 @@ int pw_batch_add(pw_batch *b, const pw_job_spec *spec, int64_t *id) {
     if (!b || !spec || !id) return PW_EINVAL;
     return guarded(&b->err, [&] {
         JobParams p = params_of(spec);
+        p.chunks = b->chunks;
         int64_t jid = b->next_id++;
         b->eng->add_job(jid, spec->n, spec->pos, spec->vort, spec->radius, spec->volume, p);
         b->status[jid] = PW_OK;
@@ -694,6 +707,7 @@ double pct(std::vector<double> v, double q) {
 struct pw_server {
     std::string err;
     std::unique_ptr<Engine> eng;
+    int chunks = 0;  // source chunks per job (pw_variant::chunks)
     size_t budget = 0, in_use = 0;
     int64_t next_id = 1;
     std::map<int64_t, SJob> jobs;
@@ -751,6 +765,7 @@ int pw_server_create(pw_server **out, int device, int kernel, size_t hbm_budget_
     return guarded(nullptr, [&] {
         auto s = std::make_unique<pw_server>();
         s->eng.reset(new Engine(device, kernel, variant_of(variant, -1, kernel)));
+        s->chunks = variant ? variant->chunks : 0;
         s->budget = hbm_budget_bytes;
         *out = s.release();
     });
@@ -776,13 +791,15 @@ int pw_server_submit(pw_server *s, const pw_job_spec *spec, size_t nsteps, doubl
         if (nsteps == 0) throw Error(PW_EINVAL, "a job needs at least one step");
         SJob j;
         j.par = params_of(spec);
+        j.par.chunks = s->chunks;
         j.id = s->next_id++;
         j.n = spec->n;
         j.nsteps = nsteps;
         const pw::Variant ev = s->eng->variant();
         const int tpt = spec->precision == PW_FP32C && ev.tpt32 ? ev.tpt32 : ev.tpt;
         j.bytes = pw::job_footprint(spec->n, spec->precision, ev.block, tpt,
-                                    spec->precision == PW_FP32C ? ev.sym32 != 0 : ev.sym64 != 0);
+                                    spec->precision == PW_FP32C ? ev.sym32 != 0 : ev.sym64 != 0,
+                                    s->chunks);
         if (j.bytes > s->budget) throw Error(PW_ENOMEM, "job larger than the HBM budget");
         j.arrival = arrival_ms < 0 ? s->now() : arrival_ms;
         j.pos.assign(spec->pos, spec->pos + 3 * spec->n);
@@ -976,54 +993,92 @@ int pw_autotune(int device, int kernel, int precision, size_t n, const double *p
     if (!best || !pos || !vort || !radius || !volume || n == 0) return PW_EINVAL;
     return guarded(nullptr, [&] {
         if (precision != PW_FP64 && precision != PW_FP32C) throw Error(PW_EINVAL, "precision");
-        // 512: binary32 symmetric pass only (binary64 jobs would run at 256)
+        // mean device time of a whole solve (pair pass + reduction)
+        auto time_solve = [&](const pw::Variant &v, int chunks) {
+            Engine eng(device, kernel, v);
+            JobParams par;
+            par.precision = precision;
+            par.chunks = chunks;
+            eng.add_job(1, n, pos, vort, radius, volume, par);
+            eng.solve();  // warm-up (plan, code object load)
+            double ms = 0.0;
+            const int r = std::max(1, reps);
+            for (int k = 0; k < r; ++k) {
+                eng.solve();
+                ms += eng.last_total_ms;
+            }
+            return ms / r;
+        };
+        auto entry = [&](int b, int t, int u, int sc, int chunks, double ms) {
+            pw_tune_entry e;
+            e.variant.block = b;
+            e.variant.targets_per_thread = t;
+            e.variant.unroll = sc == PW_SCHEME_SYMMETRIC ? 0 : u;
+            e.variant.scheme = sc;
+            e.variant.chunks = chunks;
+            e.pair_ms = ms;
+            e.pairs_per_s = (double)n * (double)(n - 1) / (ms * 1e-3);
+            return e;
+        };
+        // 1. every compiled variant at the chunk policy.  512: binary32
+        // symmetric pass only (binary64 jobs would run at 256)
         const int blocks[4] = {64, 128, 256, 512}, tpts[2] = {1, 2}, unrs[3] = {1, 2, 4};
-        // binary32: the symmetric pass (no unroll parameter) and the one-sided one
-        std::vector<int> schemes;
-        schemes = {PW_SCHEME_SYMMETRIC, PW_SCHEME_ONESIDED};
+        const int schemes[2] = {PW_SCHEME_SYMMETRIC, PW_SCHEME_ONESIDED};
         std::vector<pw_tune_entry> res;
+        std::vector<pw::Variant> vars;
         for (int sc : schemes)
-        for (int b : blocks)
-            for (int t : tpts)
-                for (int u : unrs) {
-                    const bool sym = sc == PW_SCHEME_SYMMETRIC;
-                    if (sym && u != unrs[0]) continue;
-                    if (b == 512 && (precision != PW_FP32C || !sym)) continue;
-                    pw::Variant v;
-                    v.block = b;
-                    v.tpt = t;
-                    v.sym32 = sym ? 1 : 0;
-                    v.sym64 = sym ? 1 : 0;
-                    if (!sym) {
-                        if (precision == PW_FP64)
-                            v.unroll64 = u;
-                        else
-                            v.unroll32 = u;
+            for (int b : blocks)
+                for (int t : tpts)
+                    for (int u : unrs) {
+                        const bool sym = sc == PW_SCHEME_SYMMETRIC;
+                        if (sym && u != unrs[0]) continue;
+                        if (b == 512 && (precision != PW_FP32C || !sym)) continue;
+                        pw::Variant v;
+                        v.block = b;
+                        v.tpt = t;
+                        v.sym32 = sym ? 1 : 0;
+                        v.sym64 = sym ? 1 : 0;
+                        if (!sym) {
+                            if (precision == PW_FP64)
+                                v.unroll64 = u;
+                            else
+                                v.unroll32 = u;
+                        }
+                        if (!pw::variant_exists(kernel, v.block, v.tpt, v.unroll64, v.unroll32,
+                                                v.sym32, v.sym64))
+                            continue;
+                        res.push_back(entry(b, t, u, sc, 0, time_solve(v, 0)));
+                        vars.push_back(v);
                     }
-                    if (!pw::variant_exists(kernel, v.block, v.tpt, v.unroll64, v.unroll32, v.sym32,
-                                            v.sym64))
-                        continue;
-                    Engine eng(device, kernel, v);
-                    JobParams par;
-                    par.precision = precision;
-                    eng.add_job(1, n, pos, vort, radius, volume, par);
-                    eng.solve();  // warm-up (plan, code object load)
-                    double ms = 0.0;
-                    const int r = std::max(1, reps);
-                    for (int k = 0; k < r; ++k) {
-                        eng.solve();
-                        ms += eng.last_total_ms;  // pair pass + reduction
-                    }
-                    pw_tune_entry e;
-                    e.variant.block = b;
-                    e.variant.targets_per_thread = t;
-                    e.variant.unroll = sym ? 0 : u;
-                    e.variant.scheme = sc;
-                    e.pair_ms = ms / r;
-                    e.pairs_per_s = (double)n * (double)(n - 1) / (e.pair_ms * 1e-3);
-                    res.push_back(e);
-                }
         if (res.empty()) throw Error(PW_EINVAL, "no variant to tune");
+        // 2. the fastest variant of each scheme at other chunk counts (on the
+        // symmetric pass the count is also the strip count: fewer strips mean
+        // fewer slot rows to reduce, longer items to balance)
+        const size_t nbase = res.size();
+        for (int sc : schemes) {
+            size_t k = nbase;
+            for (size_t i = 0; i < nbase; ++i)
+                if (res[i].variant.scheme == sc && (k == nbase || res[i].pair_ms < res[k].pair_ms))
+                    k = i;
+            if (k == nbase) continue;
+            const pw::Variant &v = vars[k];
+            const pw_variant &pv = res[k].variant;
+            const bool sym = sc == PW_SCHEME_SYMMETRIC;
+            const int tb = pw::block_for(pv.block, precision), tpt = pv.targets_per_thread;
+            const size_t L0 = pw::chunk_len(n, tb, tpt, sym, 0);
+            const int c0 = (int)((n + L0 - 1) / L0);
+            const size_t unit = sym ? (size_t)tb * tpt : (size_t)tb;
+            const int cmax = (int)std::min<size_t>((n + unit - 1) / unit, 4096);
+            std::vector<size_t> seen{L0};
+            for (int c : {c0 / 2, (3 * c0) / 4, (3 * c0) / 2, 2 * c0}) {
+                if (c < 1 || c > cmax) continue;
+                const size_t L = pw::chunk_len(n, tb, tpt, sym, c);
+                if (std::find(seen.begin(), seen.end(), L) != seen.end()) continue;
+                seen.push_back(L);
+                const int got = (int)((n + L - 1) / L);
+                res.push_back(entry(pv.block, tpt, pv.unroll, sc, got, time_solve(v, got)));
+            }
+        }
         size_t k = 0;
         for (size_t i = 1; i < res.size(); ++i)
             if (res[i].pair_ms < res[k].pair_ms) k = i;
@@ -1060,7 +1115,7 @@ int pw_tune_save(const char *path, int kernel, int precision, size_t n, const pw
         for (auto &l : keep) out << l << "\n";
         out << kernel << " " << precision << " " << bk << " " << e->variant.block << " "
             << e->variant.targets_per_thread << " " << e->variant.unroll << " "
-            << e->variant.scheme << " " << e->pair_ms << "\n";
+            << e->variant.scheme << " " << e->pair_ms << " " << e->variant.chunks << "\n";
     });
 }
 
@@ -1081,10 +1136,15 @@ int pw_tune_lookup(const char *path, int kernel, int precision, size_t n, pw_var
                 !pw::variant_exists(kernel, blk, tpt, precision == PW_FP64 && unr ? unr : 2,
                                     precision == PW_FP32C && unr ? unr : 4, sym32, sym64))
                 return PW_EINVAL;
+            double ms = 0.0;
+            int chunks = 0;  // absent in lines written before chunk counts were tuned
+            if (ss >> ms && !(ss >> chunks)) chunks = 0;
+            if (chunks < 0) return PW_EINVAL;
             out->block = blk;
             out->targets_per_thread = tpt;
             out->unroll = unr;
             out->scheme = sc;
+            out->chunks = chunks;
             return PW_OK;
         }
     }

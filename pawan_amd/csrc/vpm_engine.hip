@@ -76,6 +76,13 @@ size_t chunk_for_sym(size_t n, int TB, int TPT) {
     return tw * ((nblk + nstrip - 1) / nstrip);
 }
 
+size_t chunk_len(size_t n, int TB, int TPT, bool sym, int chunks) {
+    if (chunks <= 0) return sym ? chunk_for_sym(n, TB, TPT) : chunk_for(n, TB);
+    const size_t unit = sym ? (size_t)TB * TPT : (size_t)TB;
+    const size_t nunit = std::max<size_t>(1, (n + unit - 1) / unit);
+    return unit * ((nunit + (size_t)chunks - 1) / (size_t)chunks);
+}
+
 size_t strip_for_sym(size_t L, int TB, int TPT) {
     const size_t tw = (size_t)TB * TPT;
     return tw * std::max<size_t>(1, L / tw);
@@ -130,10 +137,10 @@ int block_for(int block, int precision) {
     return precision == PREC_FP64 && block > 256 ? 256 : block;
 }
 
-size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym) {
+size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym, int chunks) {
     TB = block_for(TB, precision);
     size_t npad = round_up(std::max<size_t>(n, 1), 256);
-    size_t L = sym ? chunk_for_sym(n, TB, TPT) : chunk_for(n, TB);
+    size_t L = chunk_len(std::max<size_t>(n, 1), TB, TPT, sym, chunks);
     size_t b = npad * (rec_bytes(precision) + 5 * 6 * sizeof(double) + 2 * sizeof(double)) +
                part_count(n, npad, L, (size_t)TB * TPT, sym) * sizeof(double) + 256;
     if (sym) b += jpart_count(npad, npad, strip_for_sym(L, TB, TPT)) * sizeof(double);
@@ -358,8 +365,7 @@ Job *Engine::add_job(int64_t id, size_t n, const double *pos, const double *vort
     j->kind = kind_;
     j->sym = par.precision == PREC_FP32C ? sym32_ != 0 : sym64_ != 0;
     const int tpt = tpt_of(par.precision), tb = tb_of(par.precision);
-    size_t L = chunk_override ? chunk_override
-                              : (j->sym ? chunk_for_sym(n, tb, tpt) : chunk_for(n, tb));
+    size_t L = chunk_override ? chunk_override : chunk_len(n, tb, tpt, j->sym, par.chunks);
     if (L % (size_t)tb != 0) throw Error(PW_EINVAL, "chunk must be a multiple of the block size");
     j->chunk_len = (int)L;
     j->nchunk = (int)((n + L - 1) / L);
@@ -482,7 +488,7 @@ void Engine::append(Job *j, size_t m, const double *pos, const double *vort, con
     j->n = n2;
     const int tpt = tpt_of(j->par.precision), tb = tb_of(j->par.precision);
     const size_t L = j->chunk_override ? j->chunk_override
-                                       : (j->sym ? chunk_for_sym(n2, tb, tpt) : chunk_for(n2, tb));
+                                       : chunk_len(n2, tb, tpt, j->sym, j->par.chunks);
     j->chunk_len = (int)L;
     j->nchunk = (int)((n2 + L - 1) / L);
     j->ntile = (int)((n2 + tb * tpt - 1) / (tb * tpt));
@@ -522,6 +528,7 @@ void Engine::remove_job(int64_t id) {
 
 void Engine::set_params(Job *j, const JobParams &par) {
     if (par.precision != j->par.precision) throw Error(PW_EINVAL, "precision of a job is fixed");
+    if (par.chunks != j->par.chunks) throw Error(PW_EINVAL, "chunk count of a job is fixed");
     j->par = par;
     plan_dirty_ = true;
 }

@@ -54,6 +54,7 @@ struct JobParams {
     double dt = 0.0;
     double uinf[3] = {0, 0, 0};
     int precision = PREC_FP64;
+    int chunks = 0;  // source chunks of the job (0: the chunk policy); see chunk_len
 };
 
 // Source chunk length for a job of n particles with TB targets per tile:
@@ -68,6 +69,12 @@ size_t chunk_for(size_t n, int TB);
 // and a particle's partial sums take about as many target-side rows as
 // source-side ones (~128 in all).  Depends on (n, TB, TPT) only.
 size_t chunk_for_sym(size_t n, int TB, int TPT);
+// Source chunk length of a job of n particles.  chunks > 0 asks for about that
+// many chunks: the length is rounded up to whole blocks of TB x TPT targets on
+// the symmetric pass (chunks and strips coincide there), to whole tiles of TB
+// one-sided, so the count is at most `chunks`.  chunks == 0: the policy above.
+// Depends on its arguments only.
+size_t chunk_len(size_t n, int TB, int TPT, bool sym, int chunks);
 // Strip width of a symmetric job with source chunks of L (a whole number of
 // blocks of TB x TPT targets, about L): a work item is a strip's blocks
 // against one chunk, and the strip's source-side sums fill one slot row.
@@ -75,7 +82,7 @@ size_t strip_for_sym(size_t L, int TB, int TPT);
 
 // Bytes a resident job occupies on the device (TB x TPT targets per block;
 // sym: the job uses the symmetric pass and its source-side slots).
-size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym);
+size_t job_footprint(size_t n, int precision, int TB, int TPT, bool sym, int chunks = 0);
 // Threads per workgroup of a precision's jobs in an engine built with `block`:
 // 512 exists only for the binary32 symmetric HIGHORDER pass at one target per
 // thread (half the source-side slot rows of 256), so an engine asked for 512

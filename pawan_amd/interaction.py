@@ -71,7 +71,7 @@ class Interaction:
     def __init__(self, *wakes, nu: float = 2.5e-3, kernel: int = _lib.KERNEL_HIGHORDER,
                  precision: int = _lib.FP64, uinf=(0.0, 0.0, 0.0), device: int = 0,
                  block: int = 0, chunk: int = 0, targets_per_thread: int = 0, unroll: int = 0,
-                 scheme: int = _lib.SCHEME_DEFAULT):
+                 scheme: int = _lib.SCHEME_DEFAULT, chunks: int = 0):
         L = lib()
         cfg = _lib.PwConfig()
         L.pw_config_default(C.byref(cfg))
@@ -79,7 +79,7 @@ class Interaction:
         cfg.uinf[:] = list(uinf)
         cfg.block, cfg.chunk = block, chunk
         cfg.targets_per_thread, cfg.unroll = targets_per_thread, unroll
-        cfg.scheme = scheme
+        cfg.scheme, cfg.chunks = scheme, chunks
         h = C.c_void_p()
         rc = L.pw_create(C.byref(h), C.byref(cfg))
         if rc != _lib.PW_OK:
@@ -224,11 +224,12 @@ class Interaction:
         self._size = lib().pw_size(self._h)
 
     def variant(self):
-        """The pair-pass variant this context runs (pw_get_variant)."""
+        """The pair-pass variant this context runs (pw_get_variant); chunks is
+        the source chunk count of its job."""
         v = _lib.PwVariant()
         self._ck(lib().pw_get_variant(self._h, C.byref(v)))
         return {"block": v.block, "targets_per_thread": v.targets_per_thread,
-                "unroll": v.unroll, "scheme": v.scheme}
+                "unroll": v.unroll, "scheme": v.scheme, "chunks": v.chunks}
 
     def last_timing(self):
         t, p, n = C.c_double(), C.c_double(), C.c_double()
@@ -334,9 +335,10 @@ class Batch:
     """Many independent simulations stepped in one persistent launch."""
 
     def __init__(self, device: int = 0, kernel: int = _lib.KERNEL_HIGHORDER, block: int = 0,
-                 targets_per_thread: int = 0, unroll: int = 0, scheme: int = _lib.SCHEME_DEFAULT):
+                 targets_per_thread: int = 0, unroll: int = 0, scheme: int = _lib.SCHEME_DEFAULT,
+                 chunks: int = 0):
         h = C.c_void_p()
-        v = _lib.PwVariant(block, targets_per_thread, unroll, scheme)
+        v = _lib.PwVariant(block, targets_per_thread, unroll, scheme, chunks)
         rc = lib().pw_batch_create(C.byref(h), device, kernel, C.byref(v))
         if rc != _lib.PW_OK:
             raise _lib.PawanError(rc, lib().pw_batch_last_error(None).decode())
@@ -383,13 +385,6 @@ class Batch:
     def status(self, jid):
         return lib().pw_batch_status(self._h, jid)
 
-    def variant(self):
-        """The pair-pass variant this context runs (pw_get_variant)."""
-        v = _lib.PwVariant()
-        self._ck(lib().pw_get_variant(self._h, C.byref(v)))
-        return {"block": v.block, "targets_per_thread": v.targets_per_thread,
-                "unroll": v.unroll, "scheme": v.scheme}
-
     def last_timing(self):
         t, p, n = C.c_double(), C.c_double(), C.c_double()
         dp = C.POINTER(C.c_double)
@@ -416,9 +411,9 @@ class Server:
 
     def __init__(self, hbm_budget_bytes: int, device: int = 0,
                  kernel: int = _lib.KERNEL_HIGHORDER, block: int = 0, targets_per_thread: int = 0,
-                 unroll: int = 0, scheme: int = _lib.SCHEME_DEFAULT):
+                 unroll: int = 0, scheme: int = _lib.SCHEME_DEFAULT, chunks: int = 0):
         h = C.c_void_p()
-        v = _lib.PwVariant(block, targets_per_thread, unroll, scheme)
+        v = _lib.PwVariant(block, targets_per_thread, unroll, scheme, chunks)
         rc = lib().pw_server_create(C.byref(h), device, kernel, hbm_budget_bytes, C.byref(v))
         if rc != _lib.PW_OK:
             raise _lib.PawanError(rc, lib().pw_server_last_error(None).decode())
@@ -492,8 +487,8 @@ def autotune(w: dict, precision: int = _lib.FP64, kernel: int = _lib.KERNEL_HIGH
 
     def d(e):
         return {"block": e.variant.block, "targets_per_thread": e.variant.targets_per_thread,
-                "unroll": e.variant.unroll, "scheme": e.variant.scheme, "pair_ms": e.pair_ms,
-                "pairs_per_s": e.pairs_per_s}
+                "unroll": e.variant.unroll, "scheme": e.variant.scheme,
+                "chunks": e.variant.chunks, "pair_ms": e.pair_ms, "pairs_per_s": e.pairs_per_s}
 
     if cache:
         lib().pw_tune_save(str(cache).encode(), kernel, precision, n, C.byref(best))

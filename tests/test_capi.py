@@ -87,7 +87,7 @@ def test_invalid_config_rejected_before_device_use():
     cfg = _lib.PwConfig()
     lib.pw_config_default(C.byref(cfg))
     h = C.c_void_p()
-    for field, bad in (("kernel", 7), ("precision", 5), ("block", 96)):
+    for field, bad in (("kernel", 7), ("precision", 5), ("block", 96), ("chunks", -1)):
         c2 = _lib.PwConfig.from_buffer_copy(cfg)
         setattr(c2, field, bad)
         assert lib.pw_create(C.byref(h), C.byref(c2)) == _lib.PW_EINVAL
@@ -135,4 +135,25 @@ def test_tune_cache_roundtrip(tmp_path):
     assert lib.pw_tune_save(path, 0, 1, 262144, C.byref(e4)) == 0
     assert lib.pw_tune_lookup(path, 0, 1, 262144, C.byref(v)) == 0
     assert (v.block, v.targets_per_thread, v.unroll, v.scheme) == (512, 1, 0, _lib.SCHEME_SYMMETRIC)
+    assert v.chunks == 0
     assert len(open(path.decode()).read().splitlines()) == 3
+    # a tuned chunk count is the last field of a line
+    e5 = _lib.PwTuneEntry(_lib.PwVariant(512, 1, 0, _lib.SCHEME_SYMMETRIC, 96), 60.1, 1.14e12)
+    assert lib.pw_tune_save(path, 0, 1, 262144, C.byref(e5)) == 0
+    assert lib.pw_tune_lookup(path, 0, 1, 262144, C.byref(v)) == 0
+    assert (v.block, v.scheme, v.chunks) == (512, _lib.SCHEME_SYMMETRIC, 96)
+    line = [l for l in open(path.decode()).read().splitlines() if l.startswith("0 1 18 ")]
+    assert line and line[0].split()[8] == "96"
+
+
+def test_tune_cache_line_without_chunks(tmp_path):
+    """Cache lines written before chunk counts were tuned end at ms: they
+    read as the chunk policy (chunks 0)."""
+    lib = _lib.load()
+    p = tmp_path / "old.txt"
+    p.write_text("0 0 18 256 2 2 1 117.4\n")
+    v = _lib.PwVariant(0, 0, 0, 0, 77)
+    assert lib.pw_tune_lookup(str(p).encode(), 0, 0, 262144, C.byref(v)) == 0
+    assert (v.block, v.targets_per_thread, v.unroll, v.scheme, v.chunks) == (256, 2, 2, 1, 0)
+    p.write_text("0 0 18 256 2 2 1 117.4 -3\n")
+    assert lib.pw_tune_lookup(str(p).encode(), 0, 0, 262144, C.byref(v)) == _lib.PW_EINVAL
