@@ -44,11 +44,22 @@ FLOP_PER_PAIR = 95.0          # algorithmic FLOP per ordered pair (one-sided for
 FLOP_PER_PAIR_SYM = 55.0      # the symmetric form: ~110 per unordered pair, shared by both
 FP64_VECTOR_PEAK = 78.6e12    # MI355X FP64 vector FLOP/s
 FP32_VECTOR_PEAK = 157.3e12   # MI355X FP32 vector FLOP/s (published, packed)
-# HBM bytes per binary64 pair-kernel launch at N = 262,144 (256 x 2 variant),
-# from rocprofv3 FETCH_SIZE + WRITE_SIZE passes in KiB per dispatch, scaled by
-# what the same counters read for a cold 2 GiB copy of known size (fetch x2,
-# write x1); profiles/pair_kernels_rotor_wake_262144.md
-PAIR64_TRAFFIC_262144 = (2 * 211_721 + 426_256) * 1024.0
+# HBM bytes fetched per binary64 pair-kernel launch at N = 262,144 (256 x 2
+# variant), by source chunk count: rocprofv3 FETCH_SIZE in KiB per dispatch
+# (mean over the dispatches), scaled by what the same counter reads for a cold
+# 2 GiB copy of known size (x2).  The kernel writes exactly its partial sums,
+# chunks x 13 x npad x 8 bytes (WRITE_SIZE, x1: 436 MB at 16 chunks, 873 MB at
+# 32); profiles/pair_kernels_rotor_wake_262144.md
+PAIR64_FETCH_KIB_262144 = {16: 211_721, 32: 329_701}
+
+
+def pair64_traffic(n, chunks):
+    """HBM bytes of one binary64 pair launch at N = 262,144 with `chunks`
+    source chunks (None for a size or chunk count not profiled)."""
+    if n != 262144 or chunks not in PAIR64_FETCH_KIB_262144:
+        return None
+    npad = -(-n // 256) * 256
+    return 2.0 * PAIR64_FETCH_KIB_262144[chunks] * 1024.0 + chunks * 13 * npad * 8.0
 
 
 def log(*a):
@@ -68,12 +79,15 @@ LONG_DT = 1e-5
 
 # The variants the autotuner picks for the N = 262,144 rotor wake on MI355X
 # (used with --no-tune, e.g. under a profiler): binary64 one-sided 256 x 2
-# targets, unroll 2; compensated binary32 symmetric 512 x 1 (half the
-# source-side slot rows of 256 x 1: 60.6 against 61.3 ms per RHS).
+# targets, unroll 2, 32 source chunks (116.0 against 117.2 ms per RHS at the
+# policy's 16); binary64 symmetric 256 x 1, 256 chunks and strips (97.5
+# against 98.1 at 128); compensated binary32 symmetric 256 x 1, 256 chunks
+# (55.8 against 56.7 at 128, 58.6 for 512 x 1).
 MEASURED_BEST = {
-    "fp64": {"block": 256, "targets_per_thread": 2, "unroll": 2, "scheme": 1},
-    "fp64_symmetric": {"block": 256, "targets_per_thread": 1, "unroll": 0, "scheme": 2},
-    "fp32c": {"block": 512, "targets_per_thread": 1, "unroll": 0, "scheme": 2},
+    "fp64": {"block": 256, "targets_per_thread": 2, "unroll": 2, "scheme": 1, "chunks": 32},
+    "fp64_symmetric": {"block": 256, "targets_per_thread": 1, "unroll": 0, "scheme": 2,
+                       "chunks": 256},
+    "fp32c": {"block": 256, "targets_per_thread": 1, "unroll": 0, "scheme": 2, "chunks": 256},
 }
 
 
@@ -716,8 +730,8 @@ def main():
                    "step": "one RK4 step = 4 all-pairs RHS", "dt": STEP_DT, "pairs_per_step": 4.0 * n * (n - 1)},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": FP64_VECTOR_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / FP64_VECTOR_PEAK,
-                     "traffic": (PAIR64_TRAFFIC_262144 if n == 262144 and
-                                 tuned.get("fp64", {}).get("block") == 256 and
+                     "traffic": (pair64_traffic(n, S64.variant()["chunks"])
+                                 if tuned.get("fp64", {}).get("block") == 256 and
                                  tuned.get("fp64", {}).get("targets_per_thread") == 2 else None),
                      "kernel": "pair64_kernel", "flop_per_pair": FLOP_PER_PAIR,
                      "launch_ms": r64["pair_kernel_ms_mean"]},
