@@ -3,7 +3,8 @@ updated, source-side sums in per-block slots), binary32 and binary64, against
 the CPU oracle, on the shapes where their bookkeeping can go wrong: particle
 counts around the 64-wide tiles and the target blocks, a block straddling a
 chunk boundary, several wakes, non-uniform cores, the Gaussian kernel, a
-freestream; and against the one-sided passes.  Tolerances: binary32 1e-6
+freestream, far tiles (pairs wholly in binary32) and a set far from the
+origin (where none may be far); and against the one-sided passes.  Tolerances: binary32 1e-6
 relative RMS per rate field (velocity, vorticity rate); binary64 1e-10 per
 particle, as for every result of that precision."""
 import numpy as np
@@ -186,3 +187,61 @@ def test_symmetric64_rk4_batch_alone_equals_mixed():
     ep, ev = per_particle_rel(res[0]["pos"], p), per_particle_rel(res[0]["vort"], v)
     print(f"  symmetric64 rk4 vs oracle: pos {ep:.2e} vort {ev:.2e}", flush=True)
     assert ep < 1e-11 and ev < 1e-9
+
+
+def _sparse_lattice(sub=4, cubes=4, spacing=10.0, core=0.2, seed=7):
+    """Particles `spacing` apart on a (sub x cubes)^3 lattice, ordered sub-cube
+    by sub-cube (sub^3 = 64 particles: one tile each), jittered by 10%;
+    vorticity mostly along z.  Every distance appears as often as its
+    influence decays, so the pairs between different sub-cubes carry most of
+    every velocity."""
+    g = np.stack(np.meshgrid(*[np.arange(sub)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    c = np.stack(np.meshgrid(*[np.arange(cubes)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    lat = (c[:, None, :] * sub + g[None, :, :]).reshape(-1, 3).astype(float)
+    n = lat.shape[0]
+    u = gen.uniform(seed, 6 * n).reshape(n, 6)
+    pos = (lat - (sub * cubes - 1) / 2.0 + 0.1 * (2.0 * u[:, :3] - 1.0)) * spacing
+    vort = 0.01 * np.concatenate([0.2 * (u[:, 3:5] - 0.5), 0.5 + u[:, 5:6]], -1)
+    return {"pos": np.ascontiguousarray(pos), "vort": np.ascontiguousarray(vort),
+            "radius": np.full(n, core), "volume": np.full(n, 4.0 / 3.0 * np.pi * core ** 3),
+            "n": n}
+
+
+@pytest.mark.parametrize("block", [256, 512])
+def test_far_tiles_sparse_lattice(block):
+    """A sparse lattice ordered by 64-particle sub-cubes: most sub-cubes are
+    farther from a target block than 1/8 of the coordinates, so their tiles
+    run wholly in binary32, and they carry most of every velocity; the rates
+    still meet the binary32 bar against the oracle."""
+    w = _sparse_lattice()
+    nu = 1e-3
+    print(f"case far tiles: sparse lattice N={w['n']} block={block}", flush=True)
+    vel, dv = _rates(w, nu, pa.FP32C, block=block, targets_per_thread=1)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL)
+    # the share of each velocity that comes from other sub-cubes
+    own = np.zeros_like(rv)
+    for c in range(w["n"] // 64):
+        s = slice(64 * c, 64 * c + 64)
+        own[s] = orc.rates(w["pos"][s], w["vort"][s], w["radius"][s], w["volume"][s], nu,
+                           mode=orc.SERIAL)[0]
+    share = np.linalg.norm(rv - own) / np.linalg.norm(rv)
+    ev, ed = rel_rms(vel, rv), rel_rms(dv, rd)
+    print(f"  velocity {ev:.2e} vorticity rate {ed:.2e}; other sub-cubes' share of the "
+          f"velocity {share:.2f}", flush=True)
+    assert share > 0.5
+    assert ev < TOL32 and ed < TOL32, (ev, ed)
+
+
+def test_far_test_scales_with_coordinates():
+    """A ring 1,000 radii from the origin: binary32 positions there are ~6e-5
+    apart, a thousandth of the particle spacing, so no tile of the ring may
+    count as far (its size is far below 1/8 of its coordinates); the binary32
+    rates match the oracle as they do at the origin."""
+    w = gen.vortex_ring(128, 16, R=1.0, core=0.1, seed=3, center=(1000.0, 0.0, 0.0))
+    nu = 1e-3
+    print(f"case far test: ring at x = 1000, N={w['n']}", flush=True)
+    vel, dv = _rates(w, nu, pa.FP32C)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL)
+    ev, ed = rel_rms(vel, rv), rel_rms(dv, rd)
+    print(f"  velocity {ev:.2e} vorticity rate {ed:.2e}", flush=True)
+    assert ev < TOL32 and ed < TOL32, (ev, ed)
