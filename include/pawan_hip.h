@@ -281,15 +281,19 @@ size_t pw_chunk_for(size_t n, int block);
  * retired with their final state kept until fetched.
  *
  * A pass advances resident jobs by 1, 2 or 4 RK4 stages (never past the end
- * of a job's current step).  The reference cost is the largest resident
- * job's stage, or 2e9 pair interactions if that is more.  A job whose stage
- * costs at most a quarter of it ("small") runs 4 stages in every pass; the
- * others ("large") run 2 if their stage costs at most half of it, else 1,
- * and take turns: a pass takes them in id order from a rotating cursor while
- * their work stays within one reference stage (at least one).  So small jobs
- * take whole steps per pass, no pass holds more than about one large stage
- * besides them, and every job advances.  Each stage is the same computation
- * whichever pass it falls in, so a job's result does not depend on the mix.
+ * of a job's current step), or by a slice of one stage.  The reference cost
+ * is the largest resident job's stage, held within [2e9, 1.6e10] pair
+ * interactions (a binary32 pair priced at half).  A job whose stage costs
+ * at most a quarter of it ("small") runs 4 stages in every pass; the others
+ * ("large") run 2 if their stage costs at most half of it, else 1, and a
+ * stage costing more than the reference runs in slices of about the
+ * reference's work (its pair work items cut into consecutive parts, the last
+ * part with the stage's reduction and update).  Large jobs take turns: a
+ * pass takes them in id order from a rotating cursor while their work stays
+ * within the reference (at least one).  So small jobs take whole steps per
+ * pass, no pass holds much more than the reference besides them, and every
+ * job advances.  Each stage is the same computation whichever passes it
+ * falls in, so a job's result does not depend on the mix.
  * Jobs that arrive during a pass are uploaded beside it (a second stream)
  * and join the next one; the non-finite flags of all jobs come back in one
  * copy per pass. */
@@ -320,8 +324,9 @@ int pw_server_submit(pw_server *s, const pw_job_spec *spec, size_t nsteps, doubl
                      int64_t *id);
 int pw_server_cancel(pw_server *s, int64_t id);
 /* one pass of the request loop: queue a pass (the small jobs and a turn of
- * the large ones, 1, 2 or 4 RK4 stages each), admit queued jobs that fit the
- * budget while it runs, then retire finished and failed jobs.  *running
+ * the large ones, 1, 2 or 4 RK4 stages or a stage slice each), admit queued
+ * jobs that fit the budget while it runs, then retire finished and failed
+ * jobs.  *running
  * receives the number of resident jobs after the pass. */
 int pw_server_step(pw_server *s, int64_t *running);
 /* 0 queued, 1 running, 2 done, 3 failed (non-finite), 4 cancelled, -1 unknown */

@@ -674,18 +674,24 @@ struct SJob {
     JobParams par;
     double arrival = 0.0, admitted = 0.0, finished = 0.0;
     std::vector<double> pos, vort, rad, vol;  // inputs until admitted, results when done
-    // while resident: the device job, the next RK4 stage (0-3) and the stages left
+    // while resident: the device job, the next RK4 stage (0-3) and the stages left;
+    // a stage larger than a pass's work runs in `parts` slices, `part` the next
     Job *job = nullptr;
     int stage = 0;
     size_t stages_left = 0;
+    int part = 0, parts = 1;
 };
 
 // Stages a resident job runs in one pass: 4 if its stage costs at most a
-// quarter of the pass's reference cost (the largest resident stage, or
-// kPassFloor pair interactions if that is less), 2 if at most half, else 1;
-// never past the end of its current step or of its run.  A binary32 stage is
-// priced at half a binary64 one (its pair pass runs at about twice the rate).
+// quarter of the pass's reference cost (the largest resident stage, clamped
+// to [kPassFloor, kPassCap] pair interactions), 2 if at most half, else 1;
+// never past the end of its current step or of its run.  A stage that costs
+// more than the reference runs in slices of about the reference's work, one
+// per pass.  A binary32 stage is priced at half a binary64 one (its pair pass
+// runs at about twice the rate).  kPassCap, ~28 ms of binary64 pairs, bounds
+// a pass: a 262,144-particle binary64 stage (6.9e10) runs in 5 slices.
 constexpr double kPassFloor = 2e9;
+constexpr double kPassCap = 1.6e10;
 double stage_cost(const SJob &j) {
     return (double)j.n * (double)(j.n - 1) * (j.par.precision == PW_FP32C ? 0.5 : 1.0);
 }
@@ -849,6 +855,7 @@ int pw_server_step(pw_server *s, int64_t *running) {
         std::vector<int64_t> pid;
         double ref = kPassFloor;
         for (int64_t id : s->resident) ref = std::max(ref, stage_cost(s->jobs[id]));
+        ref = std::min(ref, std::max(kPassFloor, kPassCap));
         std::vector<int64_t> large;
         for (int64_t id : s->resident) {
             SJob &j = s->jobs[id];
@@ -870,11 +877,17 @@ int pw_server_step(pw_server *s, int64_t *running) {
             for (size_t t = 0; t < large.size(); ++t) {
                 const int64_t id = large[(first + t) % large.size()];
                 SJob &j = s->jobs[id];
-                const int k = stages_this_pass(j, ref);
-                const double wk = k * stage_cost(j);
+                const double c = stage_cost(j);
+                // a stage starting now is sliced by the current reference
+                if (j.part == 0) j.parts = c > ref ? (int)std::ceil(c / ref) : 1;
+                const int k = j.parts > 1 ? 1 : stages_this_pass(j, ref);
+                const double wk = j.parts > 1 ? c / j.parts : k * c;
                 if (t > 0 && work + wk > ref) break;
                 work += wk;
-                pj.push_back(pw::PassJob{j.job, j.stage, k});
+                pw::PassJob p{j.job, j.stage, k};
+                p.part = j.part;
+                p.parts = j.parts;
+                pj.push_back(p);
                 pid.push_back(id);
                 s->cursor = id + 1;
             }
@@ -909,6 +922,13 @@ int pw_server_step(pw_server *s, int64_t *running) {
                 }
                 const size_t k = ip->second;
                 SJob &j = s->jobs[id];
+                if (pj[k].part + 1 < pj[k].parts) {  // the stage continues next time
+                    j.part++;
+                    keep.push_back(id);
+                    continue;
+                }
+                j.part = 0;
+                j.parts = 1;
                 j.stage = (j.stage + pj[k].stages) % 4;
                 j.stages_left -= (size_t)pj[k].stages;
                 j.done = (4 * j.nsteps - j.stages_left) / 4;

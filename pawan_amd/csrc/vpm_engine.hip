@@ -845,6 +845,8 @@ void Engine::pass_begin(const std::vector<PassJob> &pj, bool use_graph) {
     for (const PassJob &p : pj) {
         if (!p.job || p.stages < 1 || p.first_stage < 0 || p.first_stage + p.stages > 4)
             throw Error(PW_EINVAL, "a pass runs 1 to 4 stages of a job within one step");
+        if (p.parts < 1 || p.part < 0 || p.part >= p.parts || (p.parts > 1 && p.stages != 1))
+            throw Error(PW_EINVAL, "a stage slice is one of `parts` of a single stage");
         R = std::max(R, p.stages);
     }
     // the rounds' work: job k runs in rounds 0 .. stages - 1
@@ -853,11 +855,37 @@ void Engine::pass_begin(const std::vector<PassJob> &pj, bool use_graph) {
     std::vector<std::vector<Item>> r64(R), r32(R);
     std::vector<std::vector<CItem>> rci(R);
     size_t m64 = 0, m32 = 0, mci = 0;
+    double pairs = 0.0;
     for (int r = 0; r < R; ++r) {
-        for (size_t k = 0; k < pj.size(); ++k)
-            if (pj[k].stages > r)
-                job_items(*pj[k].job, (int)k, pj[k].job->par.precision == PREC_FP64 ? r64[r] : r32[r],
-                          rci[r]);
+        for (size_t k = 0; k < pj.size(); ++k) {
+            if (pj[k].stages <= r) continue;
+            const Job &j = *pj[k].job;
+            std::vector<Item> &dst = j.par.precision == PREC_FP64 ? r64[r] : r32[r];
+            const double pj_pairs = (double)j.n * (double)(j.n - 1);
+            if (pj[k].parts == 1) {
+                job_items(j, (int)k, dst, rci[r]);
+                pairs += pj_pairs;
+                continue;
+            }
+            // a slice: the job's items in their fixed order, cut into parts of
+            // whole rounds of the persistent grid where there are enough
+            std::vector<Item> all;
+            std::vector<CItem> ci;
+            job_items(j, (int)k, all, ci);
+            const size_t G = (size_t)(j.par.precision == PREC_FP64 ? grid64_ : grid32_);
+            size_t per = (all.size() + (size_t)pj[k].parts - 1) / (size_t)pj[k].parts;
+            if (per > G) per = round_up(per, G);
+            const size_t lo = std::min(all.size(), (size_t)pj[k].part * per);
+            const size_t hi = pj[k].part + 1 == pj[k].parts ? all.size() : std::min(all.size(), lo + per);
+            double w_all = 0.0, w = 0.0;
+            for (size_t i = 0; i < all.size(); ++i) {
+                w_all += (double)all[i].pad;
+                if (i >= lo && i < hi) w += (double)all[i].pad;
+            }
+            dst.insert(dst.end(), all.begin() + lo, all.begin() + hi);
+            if (pj[k].part + 1 == pj[k].parts) rci[r].insert(rci[r].end(), ci.begin(), ci.end());
+            pairs += w_all > 0.0 ? pj_pairs * w / w_all : 0.0;
+        }
         order_pair_items(r64[r]);
         order_pair_items(r32[r]);
         order_combine_items(rci[r], tab, (size_t)gridc_);
@@ -883,7 +911,6 @@ void Engine::pass_begin(const std::vector<PassJob> &pj, bool use_graph) {
         ++playout_;
     }
     char *h = pplan_host_.as<char>();
-    double pairs = 0.0;
     int hi = 0;
     for (int r = 0; r < R; ++r) {
         char *b = h + (size_t)r * p_round_;
@@ -900,10 +927,7 @@ void Engine::pass_begin(const std::vector<PassJob> &pj, bool use_graph) {
         pn64_[r] = cnt[0];
         pn32_[r] = cnt[1];
     }
-    for (const PassJob &p : pj) {
-        pairs += (double)p.stages * (double)p.job->n * (double)(p.job->n - 1);
-        hi = std::max(hi, p.job->slot + 1);
-    }
+    for (const PassJob &p : pj) hi = std::max(hi, p.job->slot + 1);
     PW_HIP(hipMemcpyAsync(pplan_.p, h, (size_t)R * p_round_, hipMemcpyHostToDevice, stream_));
     PW_HIP(hipEventRecord(ev_begin_, stream_));
     if (use_graph) {

@@ -146,9 +146,17 @@ size_t noalloc_scratch_bytes();
 
 // One job's share of a server pass: RK4 stages first_stage .. first_stage +
 // stages - 1 (0-based, within one step).
+// One job's share of a pass: stages first_stage .. first_stage + stages - 1
+// of its current RK4 step, or (parts > 1, one stage) slice `part` of that
+// stage's pair work items: the slices of a stage run in consecutive passes,
+// in order, and the last one runs the stage's reduction and RK4 update.
+// Which items a slice holds depends only on the job and `parts`, and every
+// item writes its own slots, so the result is the same however a stage is
+// sliced.
 struct PassJob {
     Job *job;
     int first_stage, stages;
+    int part = 0, parts = 1;
 };
 
 class Engine {

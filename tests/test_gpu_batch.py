@@ -187,6 +187,55 @@ def test_server_passes_by_stage():
     srv.close()
 
 
+@pytest.mark.parametrize("precision,n_shed", [(pa.FP64, 640), (pa.FP32C, 720)])
+def test_server_slices_a_stage_larger_than_a_pass(precision, n_shed):
+    """A stage costing more than a pass's cap (1.6e10 pair interactions, a
+    binary32 pair priced at half) runs in slices: a 163,840-particle binary64
+    or 184,320-particle binary32 rotor wake takes 2 passes per stage, 8 for
+    its step, while a 2,048-particle ring takes a whole step every pass.  The
+    wake's state equals a batch's, which runs each stage in one launch, bit
+    for bit, and the ring's equals its state served alone."""
+    big = gen.rotor_wake(n_shed)
+    small = gen.vortex_ring(128, 16, seed=4)
+    n = big["n"]
+    print(f"case server slices: rotor wake {n} precision {precision} + ring 2,048", flush=True)
+    B = pa.Batch()
+    jb0 = B.add(big, 1e-4, 1e-5, precision)
+    B.step(1)
+    ref_big = B.get(jb0)
+    B.close()
+    srv = pa.Server(64 << 30)
+    js0 = srv.submit(small, 1e-3, 0.005, 3)
+    while srv.step() > 0:
+        pass
+    ref_small = srv.fetch(js0)
+    srv.close()
+    srv = pa.Server(64 << 30)
+    jb = srv.submit(big, 1e-4, 1e-5, 1, precision)
+    js = srv.submit(small, 1e-3, 0.005, 3)
+    passes = 0
+    while True:
+        passes += 1
+        left = srv.step()
+        if passes <= 3:
+            assert srv.state(js)[1] == passes  # a whole step per pass
+        if passes < 8:
+            assert srv.state(jb) == (pa.Server.RUNNING, 0)
+        if left == 0:
+            break
+    assert passes == 8  # 4 stages x 2 slices
+    pos, vort = srv.fetch(jb)
+    assert np.array_equal(pos, ref_big["pos"]) and np.array_equal(vort, ref_big["vort"])
+    pos, vort = srv.fetch(js)
+    assert np.array_equal(pos, ref_small[0]) and np.array_equal(vort, ref_small[1])
+    st = srv.stats()
+    pairs = 4.0 * n * (n - 1) + 3 * 4.0 * 2048 * 2047
+    print(f"  passes {st['steps']}, step p50/p99 {st['step_p50_ms']:.2f}/{st['step_p99_ms']:.2f} ms, "
+          f"pairs {st['pairs']:.4e} (expected {pairs:.4e})", flush=True)
+    assert abs(st["pairs"] - pairs) < 1e-9 * pairs
+    srv.close()
+
+
 def test_server_large_jobs_take_turns():
     """Two large jobs (65,536-particle rotor wakes, one stage each the whole
     reference cost) take turns pass by pass while a small ring takes a whole
