@@ -867,21 +867,34 @@ void Engine::pass_begin(const std::vector<PassJob> &pj, bool use_graph) {
                 pairs += pj_pairs;
                 continue;
             }
-            // a slice: the job's items in their fixed order, cut into parts of
-            // whole rounds of the persistent grid where there are enough
+            // a slice: the job's items in their fixed order, cut where the
+            // running cost (Item::pad) passes q / parts of the total, each cut
+            // moved up to whole rounds of the persistent grid while every
+            // later part keeps at least a round
             std::vector<Item> all;
             std::vector<CItem> ci;
             job_items(j, (int)k, all, ci);
             const size_t G = (size_t)(j.par.precision == PREC_FP64 ? grid64_ : grid32_);
-            size_t per = (all.size() + (size_t)pj[k].parts - 1) / (size_t)pj[k].parts;
-            if (per > G) per = round_up(per, G);
-            const size_t lo = std::min(all.size(), (size_t)pj[k].part * per);
-            const size_t hi = pj[k].part + 1 == pj[k].parts ? all.size() : std::min(all.size(), lo + per);
-            double w_all = 0.0, w = 0.0;
-            for (size_t i = 0; i < all.size(); ++i) {
-                w_all += (double)all[i].pad;
-                if (i >= lo && i < hi) w += (double)all[i].pad;
+            const int parts = pj[k].parts;
+            double w_all = 0.0;
+            for (const Item &it : all) w_all += (double)it.pad;
+            std::vector<size_t> cut((size_t)parts + 1, 0);
+            cut[(size_t)parts] = all.size();
+            {
+                double acc = 0.0;
+                size_t i = 0;
+                for (int q = 1; q < parts; ++q) {
+                    const double goal = w_all * q / parts;
+                    while (i < all.size() && acc + (double)all[i].pad <= goal) acc += (double)all[i++].pad;
+                    size_t b = i;
+                    const size_t br = round_up(std::max<size_t>(b, 1), G);
+                    if (br + (size_t)(parts - q) * G <= all.size()) b = br;
+                    cut[(size_t)q] = std::max(cut[(size_t)q - 1], std::min(b, all.size()));
+                }
             }
+            const size_t lo = cut[(size_t)pj[k].part], hi = cut[(size_t)pj[k].part + 1];
+            double w = 0.0;
+            for (size_t i = lo; i < hi; ++i) w += (double)all[i].pad;
             dst.insert(dst.end(), all.begin() + lo, all.begin() + hi);
             if (pj[k].part + 1 == pj[k].parts) rci[r].insert(rci[r].end(), ci.begin(), ci.end());
             pairs += w_all > 0.0 ? pj_pairs * w / w_all : 0.0;
