@@ -44,22 +44,24 @@ FLOP_PER_PAIR = 95.0          # algorithmic FLOP per ordered pair (one-sided for
 FLOP_PER_PAIR_SYM = 55.0      # the symmetric form: ~110 per unordered pair, shared by both
 FP64_VECTOR_PEAK = 78.6e12    # MI355X FP64 vector FLOP/s
 FP32_VECTOR_PEAK = 157.3e12   # MI355X FP32 vector FLOP/s (published, packed)
-# HBM bytes fetched per binary64 pair-kernel launch at N = 262,144 (256 x 2
-# variant), by source chunk count: rocprofv3 FETCH_SIZE in KiB per dispatch
-# (mean over the dispatches), scaled by what the same counter reads for a cold
-# 2 GiB copy of known size (x2).  The kernel writes exactly its partial sums,
-# chunks x 13 x npad x 8 bytes (WRITE_SIZE, x1: 436 MB at 16 chunks, 873 MB at
-# 32); profiles/pair_kernels_rotor_wake_262144.md
-PAIR64_FETCH_KIB_262144 = {16: 211_721, 32: 329_701}
+# HBM bytes fetched per binary64 pair-kernel launch at N = 262,144 with
+# 256-thread blocks, by (targets per thread, source chunks): rocprofv3
+# FETCH_SIZE in KiB per dispatch (mean over the dispatches), scaled by what
+# the same counter reads for a cold 2 GiB copy of known size (x2).  The kernel
+# writes exactly its partial sums, chunks x 10 x npad x 8 bytes (WRITE_SIZE,
+# x1: 336 MB at 16 chunks, 671 MB at 32); profiles/pair_kernels_rotor_wake_262144.md
+PAIR64_FETCH_KIB_262144 = {(2, 32): 319_307, (2, 16): 210_465,
+                           (1, 32): 325_718, (1, 16): 218_363}
 
 
-def pair64_traffic(n, chunks):
-    """HBM bytes of one binary64 pair launch at N = 262,144 with `chunks`
-    source chunks (None for a size or chunk count not profiled)."""
-    if n != 262144 or chunks not in PAIR64_FETCH_KIB_262144:
+def pair64_traffic(n, block, targets_per_thread, chunks):
+    """HBM bytes of one binary64 pair launch at N = 262,144 for a profiled
+    variant (None otherwise)."""
+    key = (targets_per_thread, chunks)
+    if n != 262144 or block != 256 or key not in PAIR64_FETCH_KIB_262144:
         return None
     npad = -(-n // 256) * 256
-    return 2.0 * PAIR64_FETCH_KIB_262144[chunks] * 1024.0 + chunks * 13 * npad * 8.0
+    return 2.0 * PAIR64_FETCH_KIB_262144[key] * 1024.0 + chunks * 10 * npad * 8.0
 
 
 def log(*a):
@@ -657,6 +659,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (twice, fresh buffers): the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--variant64", default="", metavar="B,T,U,C",
+                    help="with --no-tune: the binary64 one-sided variant (block, targets per "
+                         "thread, unroll, chunks) instead of the measured best")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner; run the variants it picks on MI355X at N=262,144")
     ap.add_argument("--workload", default="rotor262k",
@@ -687,6 +692,9 @@ def main():
     w = gen.config_rotor_wake(args.n_shed)
     n = w["n"]
     tuned = dict(MEASURED_BEST) if args.no_tune else {}
+    if args.no_tune and args.variant64:
+        b, t, u, c = (int(x) for x in args.variant64.split(","))
+        tuned["fp64"] = {"block": b, "targets_per_thread": t, "unroll": u, "scheme": 1, "chunks": c}
     if not args.no_tune:
         for name, prec in (("fp64", pa.FP64), ("fp32c", pa.FP32C)):
             if name == "fp32c" and args.no_fp32c:
@@ -730,9 +738,8 @@ def main():
                    "step": "one RK4 step = 4 all-pairs RHS", "dt": STEP_DT, "pairs_per_step": 4.0 * n * (n - 1)},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": FP64_VECTOR_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / FP64_VECTOR_PEAK,
-                     "traffic": (pair64_traffic(n, S64.variant()["chunks"])
-                                 if tuned.get("fp64", {}).get("block") == 256 and
-                                 tuned.get("fp64", {}).get("targets_per_thread") == 2 else None),
+                     "traffic": pair64_traffic(n, **{k: S64.variant()[k] for k in
+                                                     ("block", "targets_per_thread", "chunks")}),
                      "kernel": "pair64_kernel", "flop_per_pair": FLOP_PER_PAIR,
                      "launch_ms": r64["pair_kernel_ms_mean"]},
         "variants": tuned, "autotuned": not args.no_tune,

@@ -149,21 +149,18 @@ __device__ __forceinline__ void pair64(double xi, double yi, double zi, double a
     a.v[0] = __builtin_fma(Q, cx, a.v[0]);
     a.v[1] = __builtin_fma(Q, cy, a.v[1]);
     a.v[2] = __builtin_fma(Q, cz, a.v[2]);
-    // e = a_i x a_j
+    // stretching q e + F (d.e) d, e = a_i x a_j, into one sum per component
     const double ex = __builtin_fma(aiy, s.az, -aiz * s.ay);
     const double ey = __builtin_fma(aiz, s.ax, -aix * s.az);
     const double ez = __builtin_fma(aix, s.ay, -aiy * s.ax);
-    a.v[3] = __builtin_fma(Q, ex, a.v[3]);
-    a.v[4] = __builtin_fma(Q, ey, a.v[4]);
-    a.v[5] = __builtin_fma(Q, ez, a.v[5]);
     const double fd = Fp * __builtin_fma(dz, ez, __builtin_fma(dy, ey, dx * ex));
-    a.v[6] = __builtin_fma(fd, dx, a.v[6]);
-    a.v[7] = __builtin_fma(fd, dy, a.v[7]);
-    a.v[8] = __builtin_fma(fd, dz, a.v[8]);
-    a.v[9] = __builtin_fma(Zs, s.ax, a.v[9]);
-    a.v[10] = __builtin_fma(Zs, s.ay, a.v[10]);
-    a.v[11] = __builtin_fma(Zs, s.az, a.v[11]);
-    a.v[12] = __builtin_fma(Zs, s.v, a.v[12]);
+    a.v[3] = __builtin_fma(fd, dx, __builtin_fma(Q, ex, a.v[3]));
+    a.v[4] = __builtin_fma(fd, dy, __builtin_fma(Q, ey, a.v[4]));
+    a.v[5] = __builtin_fma(fd, dz, __builtin_fma(Q, ez, a.v[5]));
+    a.v[6] = __builtin_fma(Zs, s.ax, a.v[6]);
+    a.v[7] = __builtin_fma(Zs, s.ay, a.v[7]);
+    a.v[8] = __builtin_fma(Zs, s.az, a.v[8]);
+    a.v[9] = __builtin_fma(Zs, s.v, a.v[9]);
 }
 
 struct Acc32 {
@@ -202,17 +199,14 @@ __device__ __forceinline__ void pair32(double xi, double yi, double zi, double a
     const float ex = (float)__builtin_fma(aiy, s.az, -aiz * s.ay);
     const float ey = (float)__builtin_fma(aiz, s.ax, -aix * s.az);
     const float ez = (float)__builtin_fma(aix, s.ay, -aiy * s.ax);
-    a.v[3] = __builtin_fmaf(Q, ex, a.v[3]);
-    a.v[4] = __builtin_fmaf(Q, ey, a.v[4]);
-    a.v[5] = __builtin_fmaf(Q, ez, a.v[5]);
     const float fd = Fp * __builtin_fmaf(dz, ez, __builtin_fmaf(dy, ey, dx * ex));
-    a.v[6] = __builtin_fmaf(fd, dx, a.v[6]);
-    a.v[7] = __builtin_fmaf(fd, dy, a.v[7]);
-    a.v[8] = __builtin_fmaf(fd, dz, a.v[8]);
-    a.v[9] = __builtin_fmaf(Zs, s.fx, a.v[9]);
-    a.v[10] = __builtin_fmaf(Zs, s.fy, a.v[10]);
-    a.v[11] = __builtin_fmaf(Zs, s.fz, a.v[11]);
-    a.v[12] = __builtin_fmaf(Zs, s.v, a.v[12]);
+    a.v[3] = __builtin_fmaf(fd, dx, __builtin_fmaf(Q, ex, a.v[3]));
+    a.v[4] = __builtin_fmaf(fd, dy, __builtin_fmaf(Q, ey, a.v[4]));
+    a.v[5] = __builtin_fmaf(fd, dz, __builtin_fmaf(Q, ez, a.v[5]));
+    a.v[6] = __builtin_fmaf(Zs, s.fx, a.v[6]);
+    a.v[7] = __builtin_fmaf(Zs, s.fy, a.v[7]);
+    a.v[8] = __builtin_fmaf(Zs, s.fz, a.v[8]);
+    a.v[9] = __builtin_fmaf(Zs, s.v, a.v[9]);
 }
 
 // Knuth's TwoSum: s + e == a + b exactly (round-to-nearest, no reassociation).
@@ -1223,10 +1217,10 @@ __global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ 
 // Per particle: add the chunk partials in chunk order (binary64), form the
 // rates
 //   u   = su * S[0:3] + uinf
-//   da  = sq * S[3:6] + sf * S[6:9] + sd * (v_i S[9:12] - a_i S[12])
+//   da  = sq * S[3:6] + sd * (v_i S[6:9] - a_i S[9])
 // (uniform core, positions prescaled by 1/sigma: su = sigma^-2 / kq,
-// sq = sf = sigma^-3 / kq, sd = 2 nu sigma^-5 / kz; otherwise the per-pair
-// powers of sigma are already inside and su = sq = sf = 1/kq, sd = 2 nu / kz;
+// sq = sigma^-3 / kq, sd = 2 nu sigma^-5 / kz; otherwise the per-pair
+// powers of sigma are already inside and su = sq = 1/kq, sd = 2 nu / kz;
 // kq, kz from pair_scale), then apply the RK4 / Euler stage combination of rk4.cpp:21-66
 // with unfused multiplies and adds in the reference's order, and write the
 // next stage's record.  Block 0 also re-arms the pair kernels' queues.  With
@@ -1429,9 +1423,9 @@ __global__ __launch_bounds__(256) void combine_kernel(const JobDev *__restrict__
         k[0] = J.su * S[0] + J.uinf[0];
         k[1] = J.su * S[1] + J.uinf[1];
         k[2] = J.su * S[2] + J.uinf[2];
-        k[3] = J.sq * S[3] + J.sf * S[6] + J.sd * (vi * S[9] - ax * S[12]);
-        k[4] = J.sq * S[4] + J.sf * S[7] + J.sd * (vi * S[10] - ay * S[12]);
-        k[5] = J.sq * S[5] + J.sf * S[8] + J.sd * (vi * S[11] - az * S[12]);
+        k[3] = J.sq * S[3] + J.sd * (vi * S[6] - ax * S[9]);
+        k[4] = J.sq * S[4] + J.sd * (vi * S[7] - ay * S[9]);
+        k[5] = J.sq * S[5] + J.sd * (vi * S[8] - az * S[9]);
         finish_particle(J, i, k, jmode);
     }
 }

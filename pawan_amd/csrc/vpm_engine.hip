@@ -188,11 +188,10 @@ JobDev Job::dev() const {
         d.inv_s2 = 1.0;
         d.su = (double)(1.0L / (s2 * kq));
         d.sq = (double)(1.0L / (s3 * kq));
-        d.sf = d.sq;
         d.sd = (double)(2.0L * par.nu / (s5 * kz));
     } else {
         d.inv_s2 = 0.0;
-        d.su = d.sq = d.sf = (double)(1.0L / kq);
+        d.su = d.sq = (double)(1.0L / kq);
         d.sd = (double)(2.0L * par.nu / kz);
     }
     return d;
@@ -1355,7 +1354,7 @@ void rates_noalloc(int kind, double nu, const double uinf[3], size_t n, const do
     J.nu = nu;
     for (int c = 0; c < 3; ++c) J.uinf[c] = u[c];
     const PairScale ps = pair_scale(kind, PREC_FP64);
-    J.su = J.sq = J.sf = (double)(1.0L / (long double)ps.q);
+    J.su = J.sq = (double)(1.0L / (long double)ps.q);
     J.sd = (double)(2.0L * nu / (long double)ps.z);
     const int ntile = (int)((n + NOALLOC_TB - 1) / NOALLOC_TB);
     if ((size_t)J.nchunk > NOALLOC_MAX_CHUNKS || (size_t)ntile * J.nchunk > NOALLOC_MAX_ITEMS)

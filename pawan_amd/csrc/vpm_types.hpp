@@ -10,10 +10,10 @@ constexpr int KIND_GAUSSIAN = 1;
 constexpr int PREC_FP64 = 0;
 constexpr int PREC_FP32C = 1;
 
-// Partial-sum components per (target, chunk):
-//   0-2 sum q (a_j x d)   3-5 sum q (a_i x a_j)   6-8 sum F (d.(a_i x a_j)) d
-//   9-11 sum k a_j        12 sum k v_j
-constexpr int NCOMP = 13;
+// Partial-sum components per (target, chunk) of the one-sided passes:
+//   0-2 sum q (a_j x d)   3-5 sum [q e + F (d.e) d], e = a_i x a_j (the pair's
+//   stretching vector, interaction_utils.h:69-100)   6-8 sum k a_j   9 sum k v_j
+constexpr int NCOMP = 10;
 // Symmetric binary32 pass (each unordered pair once): per (particle, slot)
 //   0-2 velocity   3-5 stretching q e + F (d.e) d   6-8 sum k a   9 sum k v
 // for the target side (slots per source chunk) and for the source side
@@ -101,7 +101,7 @@ struct JobDev {
                     // sums for particles s sw .. npad-1, at jpart_offset(s)
     double inv_s2;  // non-uniform core: unused; uniform: 1 (positions are prescaled)
     double rec_scale;  // records hold positions x rec_scale (1/sigma for a uniform core)
-    double su, sq, sf, sd;  // epilogue scales (see combine_kernel)
+    double su, sq, sd;  // epilogue scales (see combine_kernel)
     double dt, nu;
     double uinf[3];
 };

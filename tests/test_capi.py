@@ -55,7 +55,7 @@ def test_host_only_entry_points():
     # particles, 512 (512 x 1) beyond; source chunks and strips of L =
     # tw x ceil(nblk / nstrip) particles, nstrip = min(nblk, max(128, nblk / 8));
     # target-side row c holds the targets below (c + 1) L, source-side row s
-    # the particles from s L on.  The one-sided binary64 pass keeps 13 sums
+    # the particles from s L on.  The one-sided binary64 pass keeps 10 sums
     # per particle and chunk, 16 chunks.
     for n, tw in ((262144, 256), (524288, 256), (786432, 512), (65536, 256), (131072, 256),
                   (4096, 64)):
@@ -65,7 +65,7 @@ def test_host_only_entry_points():
         L = tw * -(-nblk // nstrip)
         tri = sum(min((c + 1) * L, npad) for c in range(-(-n // L)))
         rows = sum(npad - s * L for s in range(-(-npad // L)))
-        want = 16 * n + 8 * 10 * (tri + rows) - 8 * 13 * 16 * npad
+        want = 16 * n + 8 * 10 * (tri + rows) - 8 * 10 * 16 * npad
         assert lib.pw_job_footprint(n, 1) - lib.pw_job_footprint(n, 0) == want, n
     # the slots of a 262,144-particle binary32 job: under 3 GB (1.35 + 1.35)
     assert lib.pw_job_footprint(262144, 1) < 3.0e9
