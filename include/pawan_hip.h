@@ -202,7 +202,9 @@ int pw_get_variant(pw_ctx *ctx, pw_variant *out);
 
 /* device time of the last pw_solve / pw_rk4_steps / pw_euler_steps call,
  * measured with HIP events on the context's stream: total ms and the summed
- * ms of the pair-interaction kernel alone, plus the pairs it evaluated */
+ * ms of the pair-interaction kernel alone (0 after pw_rk4_steps on a context
+ * of at most 2^30 pair interactions per RHS, about 32,768 particles, whose
+ * steps replay as one captured graph each), plus the pairs it evaluated */
 int pw_last_timing(const pw_ctx *ctx, double *total_ms, double *pair_ms, double *pairs);
 
 /* ------------------------------------------------------ no allocation
@@ -269,9 +271,10 @@ int pw_batch_get(pw_batch *b, int64_t id, double *pos, double *vort, double *vel
 /* per-job status: PW_OK or PW_ENONFINITE after the last step */
 int pw_batch_status(pw_batch *b, int64_t id);
 int pw_batch_last_timing(const pw_batch *b, double *total_ms, double *pair_ms, double *pairs);
-/* source chunk length of the one-sided pass for a job with n particles (a
- * function of n and block only; jobs on the symmetric pass use 16-64 chunks,
- * a function of n, block and targets per thread) */
+/* source chunk length of the one-sided pass for a job with n particles: 16
+ * chunks, or up to one block of sources per chunk below 256 blocks (a
+ * function of n and block only; jobs on the symmetric pass use max(128,
+ * blocks / 8) chunks, capped at the block count) */
 size_t pw_chunk_for(size_t n, int block);
 
 /* ---------------------------------------------------------------- serving

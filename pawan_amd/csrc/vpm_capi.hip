@@ -336,6 +336,12 @@ int pw_get_wake(pw_ctx *ctx, size_t wake, double *pos, double *vort, double *vel
     });
 }
 
+// Contexts up to this many pair interactions per RHS (about 32,768
+// particles) replay each RK4 step as one captured graph: their kernels take
+// microseconds, and per-launch costs set the step time (a 2,048-particle
+// ring).  Larger ones launch eagerly, with the pair kernels timed one by one.
+constexpr double kGraphStepPairs = 1073741824.0;
+
 int pw_rk4_steps(pw_ctx *ctx, double dt, size_t nsteps) {
     if (!ctx) return PW_EINVAL;
     int st = PW_OK;
@@ -343,7 +349,8 @@ int pw_rk4_steps(pw_ctx *ctx, double dt, size_t nsteps) {
         if (!std::isfinite(dt)) throw Error(PW_EINVAL, "non-finite dt");
         ctx->ready();
         ctx->set_dt(dt);
-        ctx->eng->rk4(nsteps, false);
+        const double n = (double)ctx->total();
+        ctx->eng->rk4(nsteps, n * (n - 1.0) <= kGraphStepPairs);
         st = ctx->flag_status();
         if (st) ctx->err = "non-finite rates or state";
     });

@@ -219,11 +219,16 @@ __device__ __forceinline__ void two_sum_acc(float &s, float &c, float b) {
     c += e;
 }
 
-// Take the next work item from the queue; every thread of the block gets the
-// same value, so the exit below is uniform.
-__device__ __forceinline__ int next_item(int *counter, int *slot) {
+// The k-th work item a block takes: its own block index first, then the
+// queue's, which hands out gridDim.x, gridDim.x + 1, ...  A grid of no more
+// blocks than items (the engines launch no more) takes one atomic per item
+// past the first round only, and one with a block per item none.  Every
+// thread of the block gets the same value, so the exit below is uniform.
+__device__ __forceinline__ int next_item(int *counter, int *slot, int &k, int nitems) {
+    if (k++ == 0) return (int)blockIdx.x;
+    if ((int)gridDim.x >= nitems) return nitems;
     __syncthreads();
-    if (threadIdx.x == 0) *slot = atomicAdd(counter, 1);
+    if (threadIdx.x == 0) *slot = (int)gridDim.x + atomicAdd(counter, 1);
     __syncthreads();
     return *slot;
 }
@@ -246,8 +251,9 @@ __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ j
     __shared__ Rec64 tile[ST];
     __shared__ int slot;
     const int nitems = *nitems_p;
+    int taken = 0;
     for (;;) {
-        const int it = next_item(counter, &slot);
+        const int it = next_item(counter, &slot, taken, nitems);
         if (it >= nitems) break;
         const Item item = items[it];
         const JobDev &J = jobs[item.job];
@@ -341,8 +347,9 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
     __shared__ Rec32 tile[ST];
     __shared__ int slot;
     const int nitems = *nitems_p;
+    int taken = 0;
     for (;;) {
-        const int it = next_item(counter, &slot);
+        const int it = next_item(counter, &slot, taken, nitems);
         if (it >= nitems) break;
         const Item item = items[it];
         const JobDev &J = jobs[item.job];
@@ -721,9 +728,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
     __shared__ int slot;
     const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int nitems = *nitems_p;
+    int taken = 0;
     for (;;) {
         // uniform: the item's fields and its job's are scalar loads
-        const int it = __builtin_amdgcn_readfirstlane(next_item(counter, &slot));
+        const int it = __builtin_amdgcn_readfirstlane(next_item(counter, &slot, taken, nitems));
         if (it >= nitems) break;
         const Item item = items[it];
         const JobDev &J = jobs[item.job];
@@ -1094,9 +1102,10 @@ __global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ 
     __shared__ int slot;
     const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int nitems = *nitems_p;
+    int taken = 0;
     for (;;) {
         // uniform: the item's fields and its job's are scalar loads
-        const int it = __builtin_amdgcn_readfirstlane(next_item(counter, &slot));
+        const int it = __builtin_amdgcn_readfirstlane(next_item(counter, &slot, taken, nitems));
         if (it >= nitems) break;
         const Item item = items[it];
         const JobDev &J = jobs[item.job];

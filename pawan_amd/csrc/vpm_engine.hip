@@ -65,7 +65,9 @@ static int grid_for(size_t n) { return (int)std::min<size_t>(4096, (n + 255) / 2
 size_t chunk_for(size_t n, int TB) {
     if (n == 0) return (size_t)TB;
     size_t ntile = (n + TB - 1) / TB;
-    size_t nchunk = std::min<size_t>(16, ntile);
+    // 16 chunks, or enough for ~4,096 items (tiles x chunks) when there are
+    // few tiles, so that a small job still spreads over the chip
+    size_t nchunk = std::min(ntile, std::max<size_t>(16, (4096 + ntile - 1) / ntile));
     return (size_t)TB * ((ntile + nchunk - 1) / nchunk);
 }
 
@@ -614,6 +616,8 @@ void Engine::build_plan() {
     n64_ = (int)it64.size();
     n32_ = (int)it32.size();
     nci_ = (int)ci.size();
+    const int g64 = std::max(1, std::min(grid64_, n64_)), g32 = std::max(1, std::min(grid32_, n32_)),
+              gc = std::max(1, std::min(gridc_, nci_));
     // Region capacities only grow.  The region addresses are kernel arguments
     // baked into the captured step graph, so they must not move while only
     // the table contents change; a growth starts a new layout version, for
@@ -631,6 +635,13 @@ void Engine::build_plan() {
         o_cnt_ = round_up(o_ci_ + cap_ci_ * sizeof(CItem), 256);
         plan_.reset(o_cnt_ + 256);
         plan_host_.ensure(plan_.bytes);
+        ++layout_version_;
+    }
+    if (g64 != lgrid64_ || g32 != lgrid32_ || gc != lgridc_) {
+        // the grid sizes are baked into a captured step too
+        lgrid64_ = g64;
+        lgrid32_ = g32;
+        lgridc_ = gc;
         ++layout_version_;
     }
     char *h = plan_host_.as<char>();
@@ -683,9 +694,9 @@ void Engine::launch_stage(int mode, bool timed) {
             ++pair_launches;
         }
     };
-    launch_pair(PREC_FP64, grid64_, d_items64_, d_counts_ + 0, cnt + 0, n64_);
-    launch_pair(PREC_FP32C, grid32_, d_items32_, d_counts_ + 1, cnt + 1, n32_);
-    hipLaunchKernelGGL(combine_kernel, dim3(gridc_), dim3(256), 0, stream_, (const JobDev *)d_jobs_,
+    launch_pair(PREC_FP64, lgrid64_, d_items64_, d_counts_ + 0, cnt + 0, n64_);
+    launch_pair(PREC_FP32C, lgrid32_, d_items32_, d_counts_ + 1, cnt + 1, n32_);
+    hipLaunchKernelGGL(combine_kernel, dim3(lgridc_), dim3(256), 0, stream_, (const JobDev *)d_jobs_,
                        (const CItem *)d_citems_, (const int *)(d_counts_ + 2), cnt, mode);
     PW_HIP(hipGetLastError());
 }
@@ -1335,7 +1346,10 @@ void rates_noalloc(int kind, double nu, const double uinf[3], size_t n, const do
     // device), so sigma is formed per pair
     JobDev J;
     std::memset(&J, 0, sizeof J);
-    const size_t npad = round_up(n, 256), L = chunk_for(n, NOALLOC_TB);
+    // at most NOALLOC_MAX_CHUNKS chunks: the scratch's partial sums and tables
+    // are laid out for them
+    const size_t npad = round_up(n, 256),
+                 L = chunk_len(n, NOALLOC_TB, 1, false, (int)NOALLOC_MAX_CHUNKS);
     J.rec = base + NA_REC;
     J.xs = (double *)(base + NA_XS);
     J.x0 = J.xs;

@@ -58,8 +58,10 @@ struct JobParams {
 };
 
 // Source chunk length for a job of n particles with TB targets per tile:
-// at most 16 chunks per job, chunk a multiple of TB.  Depends on (n, TB) only,
-// so a job's sums are formed the same way alone or in any batch.
+// 16 chunks per job, or up to one tile per chunk when the tiles number fewer
+// than 256 (about 4,096 tile x chunk items); chunk a multiple of TB.  Depends
+// on (n, TB) only, so a job's sums are formed the same way alone or in any
+// batch.
 size_t chunk_for(size_t n, int TB);
 // Source chunk length for a job on the symmetric pass (blocks of TB x TPT
 // targets): a whole number of blocks, so that the chunks and the strips
@@ -255,6 +257,10 @@ class Engine {
     hipStream_t stream_ = nullptr;
     int cus_ = 0;
     int grid64_ = 0, grid32_ = 0, gridc_ = 0;
+    // the grids a step of the current plan launches: no more blocks than
+    // items (each block of a persistent kernel takes one queue atomic at
+    // least: a few thousand idle blocks on one counter cost tens of us)
+    int lgrid64_ = 1, lgrid32_ = 1, lgridc_ = 1;
     std::vector<std::unique_ptr<Job>> jobs_;
     bool plan_dirty_ = true;
     // plan tables: JobDev[njobs] | Item64[...] | Item32[...] | CItem[...] | ints

@@ -42,11 +42,16 @@ def test_host_only_entry_points():
     cfg = _lib.PwConfig()
     assert lib.pw_config_default(C.byref(cfg)) == 0
     assert cfg.nu == 2.5e-3 and cfg.kernel == 0 and cfg.precision == 0
-    # chunk policy: multiple of the block, at most 16 chunks, function of (n, block) only
+    # chunk policy: multiple of the block; 16 chunks, or enough for ~4,096
+    # tile x chunk items when the tiles are few; a function of (n, block) only
     for n in (1, 100, 1024, 2048, 65536, 262144, 1000003):
         for b in (64, 128, 256):
             L = lib.pw_chunk_for(n, b)
-            assert L % b == 0 and (n + L - 1) // L <= 16
+            ntile, nchunk = -(-n // b), -(-n // L)
+            assert L % b == 0 and nchunk <= max(16, -(-4096 // ntile)), (n, b, nchunk)
+            assert nchunk >= min(16, ntile) or L == b, (n, b, nchunk)
+    assert -(-2048 // lib.pw_chunk_for(2048, 64)) == 32     # one 64-source tile per chunk
+    assert -(-262144 // lib.pw_chunk_for(262144, 256)) == 16
     assert lib.pw_job_footprint(262144, 0) > 262144 * 64
     # binary32 records carry binary64 strengths too: 80 B instead of 64 B.
     # (Contexts under 8,192 particles take 64-thread blocks.)
@@ -65,7 +70,8 @@ def test_host_only_entry_points():
         L = tw * -(-nblk // nstrip)
         tri = sum(min((c + 1) * L, npad) for c in range(-(-n // L)))
         rows = sum(npad - s * L for s in range(-(-npad // L)))
-        want = 16 * n + 8 * 10 * (tri + rows) - 8 * 10 * 16 * npad
+        nch64 = -(-n // lib.pw_chunk_for(n, tw if tw < 512 else 256))
+        want = 16 * n + 8 * 10 * (tri + rows) - 8 * 10 * nch64 * npad
         assert lib.pw_job_footprint(n, 1) - lib.pw_job_footprint(n, 0) == want, n
     # the slots of a 262,144-particle binary32 job: under 3 GB (1.35 + 1.35)
     assert lib.pw_job_footprint(262144, 1) < 3.0e9
