@@ -269,6 +269,8 @@ def serving_run(pa, gen, duration_s=120.0, rate_hz=0.1, seed=1, budget=200 * 10 
     k, ids, meta = 0, [], {}
     t_end = duration_s * 1000.0
     t_log = 0.0
+    in_step = 0.0  # wall seconds spent inside pw_server_step
+    w0 = time.perf_counter()
     while True:
         if srv.now_ms() >= t_log:  # progress line every 20 s
             st = srv.stats()
@@ -283,7 +285,9 @@ def serving_run(pa, gen, duration_s=120.0, rate_hz=0.1, seed=1, budget=200 * 10 
             meta[ids[-1]] = (j["kind"], j["n"], "fp64" if j["precision"] == 0 else "fp32c",
                              j["steps"], k)
             k += 1
+        t0 = time.perf_counter()
         running = srv.step()
+        in_step += time.perf_counter() - t0
         st = srv.stats()
         if k == len(stream) and running == 0 and st["queued"] == 0:
             break
@@ -291,6 +295,7 @@ def serving_run(pa, gen, duration_s=120.0, rate_hz=0.1, seed=1, budget=200 * 10 
             break
         if running == 0 and st["queued"] == 0 and k < len(stream):
             time.sleep(max(0.0, min(stream[k]["arrival_ms"] - srv.now_ms(), 50.0)) / 1000.0)
+    wall = time.perf_counter() - w0
     st = srv.stats()
     failed, unfinished = [], []
     for i in ids:
@@ -324,7 +329,12 @@ def serving_run(pa, gen, duration_s=120.0, rate_hz=0.1, seed=1, budget=200 * 10 
         "jobs": st["submitted"], "finished": st["finished"], "failed": st["failed"],
         "step_p50_ms": st["step_p50_ms"], "step_p99_ms": st["step_p99_ms"],
         "job_p50_ms": st["job_p50_ms"], "job_p99_ms": st["job_p99_ms"],
-        "pairs_per_s": pairs_per_s,
+        # pair interactions over the device time of the passes, and over the
+        # wall time of the whole run (idle gaps between arrivals included)
+        "pairs_per_s": pairs_per_s, "pairs_per_s_wall": st["pairs"] / wall,
+        # device time of the passes over the wall time spent in pw_server_step:
+        # the rest is the host's share of a pass (planning, flags, retirement)
+        "device_busy_in_passes": st["busy_ms"] / 1000.0 / max(in_step, 1e-9),
         # the served mix at the binary64 method's 95 algorithmic FLOP per ordered
         # pair against the FP64 vector peak (binary32 jobs count at that rate too)
         "frac_fp64_peak_95flop": pairs_per_s * FLOP_PER_PAIR / FP64_VECTOR_PEAK,
