@@ -88,6 +88,33 @@ __device__ __forceinline__ void kernel64_scaled(double t, double R, double &Q, d
     Fp = __builtin_fma(-0.75, Q, Zs) * it;  // 4 kz (Zs - 3 Q) / t = kq F'
 }
 
+// ---------------------------------------------------------- far field
+// For t >= T_FAR64 (rho >= 32 sigma) the same kernel values follow from one
+// reciprocal square root and a polynomial, to binary64 rounding:
+//   HIGHORDER: (t+1.5)(t+1)^-2 = u g(u), u = 1/t, with
+//              g(u) = (1 + 1.5u)(1 + u)^-2 = sum_k (-1)^k (1 - k/2) u^k
+//              = 1 - u/2 + u^3/2 - u^4 + 3u^5/2 - 2u^6 + ...
+//              truncated after u^5 (the u^6 term is below 2e-18 relative);
+//              Zs = (1.875/pi)(t+1)^-7.5 is dropped: it changes Q and F' by
+//              less than 8 t^-6 relative (7e-18 at t = 1024) and its PSE
+//              terms are smaller than the pair's stretching terms by the
+//              same factor.
+//   GAUSSIAN : erf(sqrt(t/2)) = 1 and exp(-t/2) = 0 in binary64 (t/2 >= 512),
+//              so Q = (0.25/pi) t^-1.5 and F' = -3 Q / t.
+// Checked in multiple precision over t in [1024, 1e13]: Q and F' within
+// 1e-15 and 2.2e-15 relative (the full form: 1.1e-15 and 1.4e-15).  A pair
+// uses the far form when the bounding boxes of its target's 64-particle group
+// and its source's 64-particle group are at least sqrt(T_FAR64) sigma apart:
+// the choice depends on the particles alone, never on the kernel variant.
+constexpr float T_FAR64 = 1024.0f;
+// K_HC64-scaled coefficients of g in powers of it = 4/t (u = it/4): G_k =
+// 4 K_HC64 c_k / 4^k, c = 1, -1/2, 0, 1/2, -1, 3/2.
+constexpr double K_G0 = 4.0 * K_HC64;
+constexpr double K_G1 = -0.5 * K_HC64;
+constexpr double K_G3 = K_HC64 / 32.0;
+constexpr double K_G4 = -K_HC64 / 64.0;
+constexpr double K_G5 = 3.0 * K_HC64 / 512.0;
+
 // Binary32 kernel.  HIGHORDER is scaled by kz = kq = pi/1.875 (pair_scale):
 // the 1.875/pi of Zs moves into the epilogue.
 template <int KIND>
@@ -108,6 +135,27 @@ __device__ __forceinline__ void kernel32(float t, float rt, float &Q, float &Fp,
         Q = (phir - Zs) * it;
     }
     Fp = __builtin_fmaf(-3.0f, Q, Zs) * it;
+}
+
+// Binary32 far field (t >= T_FAR32, rho >= 11.3 sigma), scaled as kernel32:
+// HIGHORDER Q = K_HC32 g(u) t^-1/2 u^2 with g truncated after u^3 (the u^4
+// term is below 4e-9 relative) and Zs dropped (below 2e-12 relative in Q and
+// F'); GAUSSIAN Q = (0.25/pi) t^-1.5 (erf = 1 and exp(-t/2) = 0 in binary32).
+// F' = -3 Q / t for both.
+constexpr float T_FAR32 = 128.0f;
+template <int KIND>
+__device__ __forceinline__ void kernel32_far(float rt, float &Q, float &Fp) {
+    const float u = rt * rt;  // 1/t
+    if (KIND == KIND_HIGHORDER) {
+        const float u2 = u * u;
+        // K_HC32 (1 - u/2 + u^3/2)
+        const float g = __builtin_fmaf(__builtin_fmaf((float)(0.5 * K_HC32), u2, (float)(-0.5 * K_HC32)), u,
+                                       (float)K_HC32);
+        Q = (g * rt) * u2;
+    } else {
+        Q = ((float)K_CPHI * rt) * u;
+    }
+    Fp = Q * (-3.0f * u);
 }
 
 struct Acc64 {
@@ -161,6 +209,85 @@ __device__ __forceinline__ void pair64(double xi, double yi, double zi, double a
     a.v[7] = __builtin_fma(Zs, s.ay, a.v[7]);
     a.v[8] = __builtin_fma(Zs, s.az, a.v[8]);
     a.v[9] = __builtin_fma(Zs, s.v, a.v[9]);
+}
+
+// One far pair (t >= T_FAR64) of a uniform-core job in binary64: the far-
+// field kernel above, no PSE terms, and the stretching written as
+// q (e + (F'/Q)(d.e) d) with F'/Q = -3/t (HIGHORDER: -0.75 it), one
+// operation fewer.  43 arithmetic operations and one v_rsq_f64 against 54
+// and two for the full pair.
+template <int KIND>
+__device__ __forceinline__ void pair64_far(double xi, double yi, double zi, double aix, double aiy,
+                                           double aiz, const Rec64 &s, Acc64 &a) {
+    const double dx = xi - s.x, dy = yi - s.y, dz = zi - s.z;
+    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+    double Q, itm;
+    if (KIND == KIND_HIGHORDER) {
+        const double R = rsq2_nr(r2);  // 2 t^-1/2
+        const double it = R * R;       // 4 / t
+        const double it2 = it * it;
+        // G0 + G1 it + it^3 (G3 + G4 it + G5 it^2)
+        const double g = __builtin_fma(__builtin_fma(__builtin_fma(K_G5, it, K_G4), it, K_G3) * it,
+                                       it2, __builtin_fma(K_G1, it, K_G0));
+        Q = (g * R) * it2;             // kq Q
+        itm = -0.75 * it;              // F'/Q
+    } else {
+        const double rt = rsq_nr(r2);
+        const double it = rt * rt;
+        Q = (K_CPHI * rt) * it;
+        itm = -3.0 * it;
+    }
+    const double cx = __builtin_fma(s.ay, dz, -s.az * dy);
+    const double cy = __builtin_fma(s.az, dx, -s.ax * dz);
+    const double cz = __builtin_fma(s.ax, dy, -s.ay * dx);
+    a.v[0] = __builtin_fma(Q, cx, a.v[0]);
+    a.v[1] = __builtin_fma(Q, cy, a.v[1]);
+    a.v[2] = __builtin_fma(Q, cz, a.v[2]);
+    const double ex = __builtin_fma(aiy, s.az, -aiz * s.ay);
+    const double ey = __builtin_fma(aiz, s.ax, -aix * s.az);
+    const double ez = __builtin_fma(aix, s.ay, -aiy * s.ax);
+    const double k = itm * __builtin_fma(dz, ez, __builtin_fma(dy, ey, dx * ex));
+    a.v[3] = __builtin_fma(Q, __builtin_fma(k, dx, ex), a.v[3]);
+    a.v[4] = __builtin_fma(Q, __builtin_fma(k, dy, ey), a.v[4]);
+    a.v[5] = __builtin_fma(Q, __builtin_fma(k, dz, ez), a.v[5]);
+}
+
+// Smallest and largest of v over the wave's 64 lanes, in every lane.
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float uniform_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+// Bounding box of the 64 points (x, y, z) held one per lane, in binary32, in
+// wave-uniform registers: lo[0..2], hi[0..2].
+__device__ __forceinline__ void wave_box(double x, double y, double z, float *lo, float *hi) {
+    const float q[3] = {(float)x, (float)y, (float)z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        lo[c] = uniform_f(wave_min(q[c]));
+        hi[c] = uniform_f(wave_max(q[c]));
+    }
+}
+
+// Squared distance between two boxes (0 when they overlap).
+__device__ __forceinline__ float box_gap2(const float *alo, const float *ahi, const float *blo,
+                                          const float *bhi) {
+    float d2 = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float g = fmaxf(0.0f, fmaxf(blo[c] - ahi[c], alo[c] - bhi[c]));
+        d2 = __builtin_fmaf(g, g, d2);
+    }
+    return d2;
 }
 
 struct Acc32 {
@@ -233,6 +360,27 @@ __device__ __forceinline__ int next_item(int *counter, int *slot, int &k, int ni
     return *slot;
 }
 
+// Sources [j0, j0 + jn) of a staged tile against the thread's TPT targets of
+// a uniform-core job; target slot p takes them in the far form when bit p of
+// FM is set.
+template <int KIND, int TPT, int UNR, unsigned FM>
+__device__ __forceinline__ void group64(const Rec64 *tile, int j0, int jn, const Rec64 *me,
+                                        Acc64 *acc, double inv_s2) {
+#pragma unroll UNR
+    for (int j = j0; j < j0 + jn; ++j) {
+        const Rec64 sj = tile[j];
+#pragma unroll
+        for (int p = 0; p < TPT; ++p) {
+            if ((FM >> p) & 1u)
+                pair64_far<KIND>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay, me[p].az, sj,
+                                 acc[p]);
+            else
+                pair64<KIND, true>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay, me[p].az,
+                                   me[p].h, sj, inv_s2, acc[p]);
+        }
+    }
+}
+
 // ------------------------------------------------------------- pair kernels
 // Persistent: a fixed grid takes (job, target tile, source chunk) items from
 // an atomic queue until it is empty.  A workgroup of TB threads owns a tile of
@@ -240,7 +388,10 @@ __device__ __forceinline__ int next_item(int *counter, int *slot, int &k, int ni
 // sources are staged through LDS, TB records at a time, with coalesced
 // 16-byte loads, and read back as wave-uniform broadcasts, each source serving
 // TPT targets per thread.  UNR is the unroll of the source loop.  Neither TPT
-// nor UNR changes the order of any target's sums.
+// nor UNR changes the order of any target's sums.  In a uniform-core job each
+// (64-target group, 64-source group) whose bounding boxes are at least
+// sqrt(T_FAR64) apart runs in the far form: the groups are aligned to 64
+// particles of the job, so the choice is the same for every variant.
 template <int KIND, int TB, int TPT, int UNR>
 __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ jobs,
                                                     const Item *__restrict__ items,
@@ -250,6 +401,7 @@ __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ j
     constexpr int TW = TB * TPT;  // targets per tile
     __shared__ Rec64 tile[ST];
     __shared__ int slot;
+    const int lane = (int)threadIdx.x & 63;
     const int nitems = *nitems_p;
     int taken = 0;
     for (;;) {
@@ -263,12 +415,14 @@ __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ j
         int idx[TPT];
         Rec64 me[TPT];
         Acc64 acc[TPT];
+        float tlo[TPT][3], thi[TPT][3];  // box of this wave's 64 targets of slot p
 #pragma unroll
         for (int p = 0; p < TPT; ++p) {
             idx[p] = t0 + p * TB + (int)threadIdx.x;
-            me[p] = rec[idx[p] < n ? idx[p] : n - 1];
+            me[p] = rec[idx[p] < n ? idx[p] : n - 1];  // targets past n repeat particle n - 1
 #pragma unroll
             for (int m = 0; m < NCOMP; ++m) acc[p].v[m] = 0.0;
+            wave_box(me[p].x, me[p].y, me[p].z, tlo[p], thi[p]);
         }
         const double inv_s2 = J.inv_s2;
         const bool uniform = J.uniform != 0;
@@ -286,13 +440,30 @@ __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ j
             const bool diag = (s0 < t0 + TW) && (t0 < s0 + cnt);
             if (!diag) {
                 if (uniform) {
-#pragma unroll UNR
-                    for (int j = 0; j < cnt; ++j) {
-                        const Rec64 sj = tile[j];
+                    // 64 sources at a time; target slot p takes them in the
+                    // far form when its group's box and theirs are far apart
+                    for (int g0 = 0; g0 < cnt; g0 += 64) {
+                        const int gn = min(64, cnt - g0);
+                        unsigned fm = 0;
+                        if (gn == 64) {
+                            const Rec64 &q = tile[g0 + lane];
+                            float slo[3], shi[3];
+                            wave_box(q.x, q.y, q.z, slo, shi);
 #pragma unroll
-                        for (int p = 0; p < TPT; ++p)
-                            pair64<KIND, true>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
-                                               me[p].az, me[p].h, sj, inv_s2, acc[p]);
+                            for (int p = 0; p < TPT; ++p)
+                                if (box_gap2(tlo[p], thi[p], slo, shi) >= T_FAR64) fm |= 1u << p;
+                            fm = __builtin_amdgcn_readfirstlane(fm);
+                        }
+                        if (fm == 0)
+                            group64<KIND, TPT, UNR, 0>(tile, g0, gn, me, acc, inv_s2);
+                        else if (fm == (1u << TPT) - 1)
+                            group64<KIND, TPT, UNR, (1u << TPT) - 1>(tile, g0, gn, me, acc, inv_s2);
+                        else if constexpr (TPT == 2) {
+                            if (fm == 1)
+                                group64<KIND, TPT, UNR, 1>(tile, g0, gn, me, acc, inv_s2);
+                            else
+                                group64<KIND, TPT, UNR, 2>(tile, g0, gn, me, acc, inv_s2);
+                        }
                     }
                 } else {
 #pragma unroll UNR
@@ -565,14 +736,19 @@ __device__ __forceinline__ Part32 load_src(const SrcTile &t, int k, Part32) {
 // One unordered pair (target me, source s): target-side terms added to a,
 // source-side terms stored (FIRST) or added to b.  MASKED: a pair with
 // valid == false contributes exactly zero (its operands are finite).
-template <int KIND, bool UNIFORM, bool MASKED, bool FIRST, class PT>
+// DIFF == false: a far pair of a uniform-core job (t >= T_FAR32): the far-
+// field kernel and no PSE terms (sums 6-9 untouched on both sides).
+template <int KIND, bool UNIFORM, bool MASKED, bool FIRST, bool DIFF = true, class PT>
 __device__ __forceinline__ void pair32s(const PT &me, const PT &s, bool valid, Acc32s &a,
                                         Acc32s &b) {
+    static_assert(DIFF || (UNIFORM && !MASKED), "the far form is for full tiles of uniform jobs");
     const float3 d = disp32(me, s);
     const float dx = d.x, dy = d.y, dz = d.z;
     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-    float Q, Fp, Zs;
-    if (UNIFORM) {
+    float Q, Fp, Zs = 0.0f;
+    if (!DIFF) {
+        kernel32_far<KIND>(__builtin_amdgcn_rsqf(r2), Q, Fp);
+    } else if (UNIFORM) {
         const float t = MASKED ? (valid ? r2 : 1.0f) : r2;
         kernel32<KIND>(t, __builtin_amdgcn_rsqf(t), Q, Fp, Zs);
     } else {
@@ -610,10 +786,12 @@ __device__ __forceinline__ void pair32s(const PT &me, const PT &s, bool valid, A
     a.v[4] += sy;
     a.v[5] += sz;
     // PSE sums: i gets k a_j, k v_j; j gets k a_i, k v_i
-    a.v[6] = __builtin_fmaf(Zs, s.fx, a.v[6]);
-    a.v[7] = __builtin_fmaf(Zs, s.fy, a.v[7]);
-    a.v[8] = __builtin_fmaf(Zs, s.fz, a.v[8]);
-    a.v[9] = __builtin_fmaf(Zs, s.v, a.v[9]);
+    if (DIFF) {
+        a.v[6] = __builtin_fmaf(Zs, s.fx, a.v[6]);
+        a.v[7] = __builtin_fmaf(Zs, s.fy, a.v[7]);
+        a.v[8] = __builtin_fmaf(Zs, s.fz, a.v[8]);
+        a.v[9] = __builtin_fmaf(Zs, s.v, a.v[9]);
+    }
     if (FIRST) {
         b.v[0] = __builtin_fmaf(uy, me.fz, -uz * me.fy);
         b.v[1] = __builtin_fmaf(uz, me.fx, -ux * me.fz);
@@ -621,10 +799,12 @@ __device__ __forceinline__ void pair32s(const PT &me, const PT &s, bool valid, A
         b.v[3] = sx;
         b.v[4] = sy;
         b.v[5] = sz;
-        b.v[6] = Zs * me.fx;
-        b.v[7] = Zs * me.fy;
-        b.v[8] = Zs * me.fz;
-        b.v[9] = Zs * me.v;
+        if (DIFF) {
+            b.v[6] = Zs * me.fx;
+            b.v[7] = Zs * me.fy;
+            b.v[8] = Zs * me.fz;
+            b.v[9] = Zs * me.v;
+        }
     } else {
         b.v[0] = __builtin_fmaf(uy, me.fz, __builtin_fmaf(-uz, me.fy, b.v[0]));
         b.v[1] = __builtin_fmaf(uz, me.fx, __builtin_fmaf(-ux, me.fz, b.v[1]));
@@ -632,10 +812,12 @@ __device__ __forceinline__ void pair32s(const PT &me, const PT &s, bool valid, A
         b.v[3] += sx;
         b.v[4] += sy;
         b.v[5] += sz;
-        b.v[6] = __builtin_fmaf(Zs, me.fx, b.v[6]);
-        b.v[7] = __builtin_fmaf(Zs, me.fy, b.v[7]);
-        b.v[8] = __builtin_fmaf(Zs, me.fz, b.v[8]);
-        b.v[9] = __builtin_fmaf(Zs, me.v, b.v[9]);
+        if (DIFF) {
+            b.v[6] = __builtin_fmaf(Zs, me.fx, b.v[6]);
+            b.v[7] = __builtin_fmaf(Zs, me.fy, b.v[7]);
+            b.v[8] = __builtin_fmaf(Zs, me.fz, b.v[8]);
+            b.v[9] = __builtin_fmaf(Zs, me.v, b.v[9]);
+        }
     }
 }
 
@@ -643,8 +825,10 @@ __device__ __forceinline__ void pair32s(const PT &me, const PT &s, bool valid, A
 // MASKED: per-target-slot state: live[p] (slot p pairs with this tile at
 // all), nt[p] (targets of the slot's 64 that exist), diag[p] (the tile is
 // the slot's own: only j > i); cnt sources exist in the tile.
-// PT: Part32, or Part32f for a far tile (never masked).
-template <int KIND, bool UNIFORM, bool MASKED, int TPT, int SB, class PT>
+// PT: Part32, or Part32f for a far tile (never masked).  DIFF == false: a
+// far tile in the far-field form; sums 6-9 are neither added nor rotated (64
+// rotations of a sum that receives nothing leave it where it was).
+template <int KIND, bool UNIFORM, bool MASKED, int TPT, int SB, bool DIFF = true, class PT>
 __device__ __forceinline__ void sym_steps(const SrcTile &tile, int s_begin, int lane, const PT *me,
                                           const bool *live, const int *nt, const bool *diag,
                                           int cnt, Acc32s *a, Acc32s &jacc) {
@@ -656,9 +840,10 @@ __device__ __forceinline__ void sym_steps(const SrcTile &tile, int s_begin, int 
         const PT sj = load_src(tile, e0 + u, PT{});
         Acc32s b;
         if (!MASKED) {
-            pair32s<KIND, UNIFORM, false, true>(me[0], sj, true, a[0], b);
+            pair32s<KIND, UNIFORM, false, true, DIFF>(me[0], sj, true, a[0], b);
 #pragma unroll
-            for (int p = 1; p < TPT; ++p) pair32s<KIND, UNIFORM, false, false>(me[p], sj, true, a[p], b);
+            for (int p = 1; p < TPT; ++p)
+                pair32s<KIND, UNIFORM, false, false, DIFF>(me[p], sj, true, a[p], b);
         } else {
             const bool src_ok = k < cnt;
 #pragma unroll
@@ -672,26 +857,16 @@ __device__ __forceinline__ void sym_steps(const SrcTile &tile, int s_begin, int 
             }
         }
 #pragma unroll
-        for (int m = 0; m < NCOMP_S; ++m) jacc.v[m] = rot_add(jacc.v[m], b.v[m]);
+        for (int m = 0; m < (DIFF ? NCOMP_S : 6); ++m) jacc.v[m] = rot_add(jacc.v[m], b.v[m]);
     }
 }
 
-__device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-    return v;
-}
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-    return v;
-}
-
 // Whether a tile whose box is [tlo, thi] is far from a block whose box is
-// [blo, bhi] (K_FAR).
+// [blo, bhi] (K_FAR); d2 receives the squared distance between the boxes.
 __device__ __forceinline__ bool far_boxes(const float *blo, const float *bhi, const float *tlo,
-                                          const float *thi) {
-    float d2 = 0.0f, m = 0.0f;
+                                          const float *thi, float &d2) {
+    float m = 0.0f;
+    d2 = 0.0f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float g = fmaxf(0.0f, fmaxf(tlo[c] - bhi[c], blo[c] - thi[c]));
@@ -724,7 +899,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
     __shared__ double jl[NW][NCOMP_S][64];  // per-source binary64 sums, per wave
     __shared__ double prev[NCOMP_S][64];    // the strip row's sums before this block
     __shared__ float bbw[NW][6];            // per wave: box of its targets (lo xyz, hi xyz)
-    __shared__ int tfar;                    // the staged tile is far from the block
+    __shared__ int tfar;                    // the staged tile is far from the block (1),
+                                            // and t >= T_FAR32 for all its pairs (2)
     __shared__ int slot;
     const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int nitems = *nitems_p;
@@ -836,7 +1012,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                             blo[c] = fminf(blo[c], bbw[ww][c]);
                             bhi[c] = fmaxf(bhi[c], bbw[ww][3 + c]);
                         }
-                    f = far_boxes(blo, bhi, tlo, thi) ? 1 : 0;
+                    float d2;
+                    f = far_boxes(blo, bhi, tlo, thi, d2) ? 1 : 0;
+                    // uniform core: coordinates are in units of sigma, t = rho^2
+                    if (f && uniform && d2 >= T_FAR32) f = 2;
                 }
                 if (k == 0) tfar = f;
             }
@@ -847,7 +1026,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                     if (j < cnt) prev[m][j] = jrow[(size_t)m * len + (size_t)(s0 - S0 + j)];
                 }
             __syncthreads();
-            const bool far = __builtin_amdgcn_readfirstlane(tfar) != 0;
+            const int farv = __builtin_amdgcn_readfirstlane(tfar);
+            const bool far = farv != 0, farz = farv == 2;
             bool live[TPT], diag[TPT], full = cnt == 64, any = false;
 #pragma unroll
             for (int p = 0; p < TPT; ++p) {
@@ -879,7 +1059,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                             for (int p = 0; p < TPT; ++p)
                                 mf[p] = Part32f{(float)me[p].x, (float)me[p].y, (float)me[p].z,
                                                 me[p].fx, me[p].fy, me[p].fz, me[p].v, me[p].h};
-                            if (uniform)
+                            if (farz)
+                                sym_steps<KIND, true, false, TPT, SBJ, false>(tile, sj, lane, mf, live,
+                                                                              nt, diag, cnt, a, jacc);
+                            else if (uniform)
                                 sym_steps<KIND, true, false, TPT, SBJ>(tile, sj, lane, mf, live, nt,
                                                                        diag, cnt, a, jacc);
                             else
@@ -910,8 +1093,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                         // their order are those of the plain update: bitwise the same
                         // rates, 0.35% faster at N = 262,144)
                         for (int m = 0; m < NCOMP_S; ++m)
-                            __hip_atomic_fetch_add(&jl[w][m][k], (double)jacc.v[m], __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+                            if (m < 6 || !(full && farz))  // far-field runs add no PSE sums
+                                __hip_atomic_fetch_add(&jl[w][m][k], (double)jacc.v[m], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
                         // the next run's lanes update entries this run's
                         // lanes wrote
                         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
