@@ -211,17 +211,10 @@ __device__ __forceinline__ void pair64(double xi, double yi, double zi, double a
     a.v[9] = __builtin_fma(Zs, s.v, a.v[9]);
 }
 
-// One far pair (t >= T_FAR64) of a uniform-core job in binary64: the far-
-// field kernel above, no PSE terms, and the stretching written as
-// q (e + (F'/Q)(d.e) d) with F'/Q = -3/t (HIGHORDER: -0.75 it), one
-// operation fewer.  43 arithmetic operations and one v_rsq_f64 against 54
-// and two for the full pair.
+// The far-field kernel at t = r2 (uniform core, t >= T_FAR64), scaled as the
+// full form of each kernel is: Q, and itm = F'/Q = -3/t.
 template <int KIND>
-__device__ __forceinline__ void pair64_far(double xi, double yi, double zi, double aix, double aiy,
-                                           double aiz, const Rec64 &s, Acc64 &a) {
-    const double dx = xi - s.x, dy = yi - s.y, dz = zi - s.z;
-    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-    double Q, itm;
+__device__ __forceinline__ void kernel64_far(double r2, double &Q, double &itm) {
     if (KIND == KIND_HIGHORDER) {
         const double R = rsq2_nr(r2);  // 2 t^-1/2
         const double it = R * R;       // 4 / t
@@ -237,6 +230,19 @@ __device__ __forceinline__ void pair64_far(double xi, double yi, double zi, doub
         Q = (K_CPHI * rt) * it;
         itm = -3.0 * it;
     }
+}
+
+// One far pair (t >= T_FAR64) of a uniform-core job in binary64: the far-
+// field kernel, no PSE terms, and the stretching written as
+// q (e + (F'/Q)(d.e) d), one operation fewer.  43 arithmetic operations and
+// one v_rsq_f64 against 54 and two for the full pair.
+template <int KIND>
+__device__ __forceinline__ void pair64_far(double xi, double yi, double zi, double aix, double aiy,
+                                           double aiz, const Rec64 &s, Acc64 &a) {
+    const double dx = xi - s.x, dy = yi - s.y, dz = zi - s.z;
+    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+    double Q, itm;
+    kernel64_far<KIND>(r2, Q, itm);
     const double cx = __builtin_fma(s.ay, dz, -s.az * dy);
     const double cy = __builtin_fma(s.az, dx, -s.ax * dz);
     const double cz = __builtin_fma(s.ax, dy, -s.ay * dx);
@@ -360,24 +366,46 @@ __device__ __forceinline__ int next_item(int *counter, int *slot, int &k, int ni
     return *slot;
 }
 
-// Sources [j0, j0 + jn) of a staged tile against the thread's TPT targets of
-// a uniform-core job; target slot p takes them in the far form when bit p of
-// FM is set.
-template <int KIND, int TPT, int UNR, unsigned FM>
+// Sources [j0, j0 + jn) of a staged tile (first source s0) against the
+// thread's TPT targets of a uniform-core job; target slot p takes them in the
+// far form when bit p of FM is set.  DIAG: the tile holds some of the block's
+// own targets, and a pair j == i is skipped (a far group is never the
+// target's own).
+template <int KIND, int TPT, int UNR, unsigned FM, bool DIAG>
 __device__ __forceinline__ void group64(const Rec64 *tile, int j0, int jn, const Rec64 *me,
-                                        Acc64 *acc, double inv_s2) {
+                                        Acc64 *acc, double inv_s2, int s0, const int *idx) {
 #pragma unroll UNR
     for (int j = j0; j < j0 + jn; ++j) {
         const Rec64 sj = tile[j];
 #pragma unroll
         for (int p = 0; p < TPT; ++p) {
-            if ((FM >> p) & 1u)
+            if ((FM >> p) & 1u) {
                 pair64_far<KIND>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay, me[p].az, sj,
                                  acc[p]);
-            else
+            } else {
+                if (DIAG && s0 + j == idx[p]) continue;
                 pair64<KIND, true>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay, me[p].az,
                                    me[p].h, sj, inv_s2, acc[p]);
+            }
         }
+    }
+}
+
+// group64 for the far mask fm (one bit per target slot, wave-uniform).
+template <int KIND, int TPT, int UNR, bool DIAG>
+__device__ __forceinline__ void group64_fm(unsigned fm, const Rec64 *tile, int j0, int jn,
+                                           const Rec64 *me, Acc64 *acc, double inv_s2, int s0,
+                                           const int *idx) {
+    constexpr unsigned ALL = (1u << TPT) - 1;
+    if (fm == 0)
+        group64<KIND, TPT, UNR, 0, DIAG>(tile, j0, jn, me, acc, inv_s2, s0, idx);
+    else if (fm == ALL)
+        group64<KIND, TPT, UNR, ALL, DIAG>(tile, j0, jn, me, acc, inv_s2, s0, idx);
+    else if constexpr (TPT == 2) {
+        if (fm == 1)
+            group64<KIND, TPT, UNR, 1, DIAG>(tile, j0, jn, me, acc, inv_s2, s0, idx);
+        else
+            group64<KIND, TPT, UNR, 2, DIAG>(tile, j0, jn, me, acc, inv_s2, s0, idx);
     }
 }
 
@@ -437,57 +465,46 @@ __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ j
                 for (int k = threadIdx.x; k < cnt * 4; k += TB) dst[k] = src[k];
             }
             __syncthreads();
+            // a tile holding some of the block's own targets skips j == i
             const bool diag = (s0 < t0 + TW) && (t0 < s0 + cnt);
-            if (!diag) {
-                if (uniform) {
-                    // 64 sources at a time; target slot p takes them in the
-                    // far form when its group's box and theirs are far apart
-                    for (int g0 = 0; g0 < cnt; g0 += 64) {
-                        const int gn = min(64, cnt - g0);
-                        unsigned fm = 0;
-                        if (gn == 64) {
-                            const Rec64 &q = tile[g0 + lane];
-                            float slo[3], shi[3];
-                            wave_box(q.x, q.y, q.z, slo, shi);
-#pragma unroll
-                            for (int p = 0; p < TPT; ++p)
-                                if (box_gap2(tlo[p], thi[p], slo, shi) >= T_FAR64) fm |= 1u << p;
-                            fm = __builtin_amdgcn_readfirstlane(fm);
-                        }
-                        if (fm == 0)
-                            group64<KIND, TPT, UNR, 0>(tile, g0, gn, me, acc, inv_s2);
-                        else if (fm == (1u << TPT) - 1)
-                            group64<KIND, TPT, UNR, (1u << TPT) - 1>(tile, g0, gn, me, acc, inv_s2);
-                        else if constexpr (TPT == 2) {
-                            if (fm == 1)
-                                group64<KIND, TPT, UNR, 1>(tile, g0, gn, me, acc, inv_s2);
-                            else
-                                group64<KIND, TPT, UNR, 2>(tile, g0, gn, me, acc, inv_s2);
-                        }
-                    }
-                } else {
-#pragma unroll UNR
-                    for (int j = 0; j < cnt; ++j) {
-                        const Rec64 sj = tile[j];
+            if (uniform) {
+                // 64 sources at a time; target slot p takes them in the far
+                // form when its group's box and theirs are far apart (never
+                // its own group: the gap is 0)
+                for (int g0 = 0; g0 < cnt; g0 += 64) {
+                    const int gn = min(64, cnt - g0);
+                    unsigned fm = 0;
+                    if (gn == 64) {
+                        const Rec64 &q = tile[g0 + lane];
+                        float slo[3], shi[3];
+                        wave_box(q.x, q.y, q.z, slo, shi);
 #pragma unroll
                         for (int p = 0; p < TPT; ++p)
-                            pair64<KIND, false>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
-                                                me[p].az, me[p].h, sj, inv_s2, acc[p]);
+                            if (box_gap2(tlo[p], thi[p], slo, shi) >= T_FAR64) fm |= 1u << p;
+                        fm = __builtin_amdgcn_readfirstlane(fm);
                     }
+                    if (diag)
+                        group64_fm<KIND, TPT, UNR, true>(fm, tile, g0, gn, me, acc, inv_s2, s0, idx);
+                    else
+                        group64_fm<KIND, TPT, UNR, false>(fm, tile, g0, gn, me, acc, inv_s2, s0, idx);
+                }
+            } else if (!diag) {
+#pragma unroll UNR
+                for (int j = 0; j < cnt; ++j) {
+                    const Rec64 sj = tile[j];
+#pragma unroll
+                    for (int p = 0; p < TPT; ++p)
+                        pair64<KIND, false>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
+                                            me[p].az, me[p].h, sj, inv_s2, acc[p]);
                 }
             } else {
-                // a tile holding some of the block's own targets: skip j == i
                 for (int j = 0; j < cnt; ++j) {
                     const Rec64 sj = tile[j];
 #pragma unroll
                     for (int p = 0; p < TPT; ++p) {
                         if (s0 + j == idx[p]) continue;
-                        if (uniform)
-                            pair64<KIND, true>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
-                                               me[p].az, me[p].h, sj, inv_s2, acc[p]);
-                        else
-                            pair64<KIND, false>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
-                                                me[p].az, me[p].h, sj, inv_s2, acc[p]);
+                        pair64<KIND, false>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
+                                            me[p].az, me[p].h, sj, inv_s2, acc[p]);
                     }
                 }
             }
@@ -1178,14 +1195,20 @@ __device__ __forceinline__ Part64 load_src64(const SrcTile64 &t, int k) {
 
 // One unordered pair in binary64: target-side terms added to a, source-side
 // terms added to b (the source's running sums, just rotated in).  MASKED:
-// valid == false contributes exactly zero.
-template <int KIND, bool UNIFORM, bool MASKED>
+// valid == false contributes exactly zero.  DIFF == false: a far pair of a
+// uniform-core job (t >= T_FAR64): the far-field kernel, no PSE terms.
+template <int KIND, bool UNIFORM, bool MASKED, bool DIFF = true>
 __device__ __forceinline__ void pair64s(const Part64 &me, const Part64 &s, bool valid, Acc64s &a,
                                         Acc64s &b) {
+    static_assert(DIFF || (UNIFORM && !MASKED), "the far form is for full tiles of uniform jobs");
     const double dx = me.x - s.x, dy = me.y - s.y, dz = me.z - s.z;
     const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-    double Q, Fp, Zs;
-    if (UNIFORM) {
+    double Q, Fp, Zs = 0.0;
+    if (!DIFF) {
+        double itm;
+        kernel64_far<KIND>(r2, Q, itm);
+        Fp = Q * itm;
+    } else if (UNIFORM) {
         const double t = MASKED ? (valid ? r2 : 1.0) : r2;
         if (KIND == KIND_HIGHORDER)
             kernel64_scaled(t, rsq2_nr(t), Q, Fp, Zs);
@@ -1225,23 +1248,29 @@ __device__ __forceinline__ void pair64s(const Part64 &me, const Part64 &s, bool 
     a.v[3] += sx;
     a.v[4] += sy;
     a.v[5] += sz;
-    a.v[6] = __builtin_fma(Zs, s.ax, a.v[6]);
-    a.v[7] = __builtin_fma(Zs, s.ay, a.v[7]);
-    a.v[8] = __builtin_fma(Zs, s.az, a.v[8]);
-    a.v[9] = __builtin_fma(Zs, s.v, a.v[9]);
+    if (DIFF) {
+        a.v[6] = __builtin_fma(Zs, s.ax, a.v[6]);
+        a.v[7] = __builtin_fma(Zs, s.ay, a.v[7]);
+        a.v[8] = __builtin_fma(Zs, s.az, a.v[8]);
+        a.v[9] = __builtin_fma(Zs, s.v, a.v[9]);
+    }
     b.v[0] = __builtin_fma(uy, me.az, __builtin_fma(-uz, me.ay, b.v[0]));
     b.v[1] = __builtin_fma(uz, me.ax, __builtin_fma(-ux, me.az, b.v[1]));
     b.v[2] = __builtin_fma(ux, me.ay, __builtin_fma(-uy, me.ax, b.v[2]));
     b.v[3] += sx;
     b.v[4] += sy;
     b.v[5] += sz;
-    b.v[6] = __builtin_fma(Zs, me.ax, b.v[6]);
-    b.v[7] = __builtin_fma(Zs, me.ay, b.v[7]);
-    b.v[8] = __builtin_fma(Zs, me.az, b.v[8]);
-    b.v[9] = __builtin_fma(Zs, me.v, b.v[9]);
+    if (DIFF) {
+        b.v[6] = __builtin_fma(Zs, me.ax, b.v[6]);
+        b.v[7] = __builtin_fma(Zs, me.ay, b.v[7]);
+        b.v[8] = __builtin_fma(Zs, me.az, b.v[8]);
+        b.v[9] = __builtin_fma(Zs, me.v, b.v[9]);
+    }
 }
 
-template <int KIND, bool UNIFORM, bool MASKED, int TPT, int SB>
+// DIFF == false: a far tile in the far-field form; sums 6-9 stay untouched
+// (and unrotated: they receive nothing in this tile).
+template <int KIND, bool UNIFORM, bool MASKED, int TPT, int SB, bool DIFF = true>
 __device__ __forceinline__ void sym64_steps(const SrcTile64 &tile, int s_begin, int lane,
                                             const Part64 *me, const bool *live, const int *nt,
                                             const bool *diag, int cnt, Acc64s *a,
@@ -1254,10 +1283,10 @@ __device__ __forceinline__ void sym64_steps(const SrcTile64 &tile, int s_begin, 
         const Part64 sj = load_src64(tile, e0 + u);
         Acc64s b;
 #pragma unroll
-        for (int m = 0; m < NCOMP_S; ++m) b.v[m] = rot64(jacc.v[m]);
+        for (int m = 0; m < NCOMP_S; ++m) b.v[m] = (DIFF || m < 6) ? rot64(jacc.v[m]) : jacc.v[m];
         if (!MASKED) {
 #pragma unroll
-            for (int p = 0; p < TPT; ++p) pair64s<KIND, UNIFORM, false>(me[p], sj, true, a[p], b);
+            for (int p = 0; p < TPT; ++p) pair64s<KIND, UNIFORM, false, DIFF>(me[p], sj, true, a[p], b);
         } else {
             const bool src_ok = k < cnt;
 #pragma unroll
@@ -1283,6 +1312,8 @@ __global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ 
     __shared__ SrcTile64 tile;
     __shared__ double jl[NW][NCOMP_S][64];
     __shared__ double prev[NCOMP_S][64];  // the strip row's sums before this block
+    __shared__ float bbw[NW][6];          // per wave: box of its targets (lo xyz, hi xyz)
+    __shared__ int tfar;                  // the staged tile's pairs all have t >= T_FAR64
     __shared__ int slot;
     const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int nitems = *nitems_p;
@@ -1320,6 +1351,25 @@ __global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ 
 #pragma unroll
             for (int m = 0; m < NCOMP_S; ++m) a[p].v[m] = 0.0;
         }
+        {  // this wave's share of the block's box (targets past n repeat particle n - 1)
+            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+            for (int p = 0; p < TPT; ++p) {
+                float plo[3], phi[3];
+                wave_box(me[p].x, me[p].y, me[p].z, plo, phi);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    lo[c] = fminf(lo[c], plo[c]);
+                    hi[c] = fmaxf(hi[c], phi[c]);
+                }
+            }
+            if (lane == 0)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    bbw[w][c] = lo[c];
+                    bbw[w][3 + c] = hi[c];
+                }
+        }
         const bool uniform = J.uniform != 0;
         for (int s0 = c0; s0 < c1; s0 += 64) {
             const int cnt = min(64, c1 - s0);
@@ -1341,6 +1391,27 @@ __global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ 
                     tile.v[k + c] = r.v;
                     tile.h[k + c] = r.h;
                 }
+                // far test (full tiles of uniform-core jobs, whose coordinates
+                // are in units of sigma): the tile's box against the block's
+                int f = 0;
+                if (cnt == 64 && uniform) {
+                    float tlo[3], thi[3], blo[3], bhi[3];
+                    wave_box(r.x, r.y, r.z, tlo, thi);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        blo[c] = bbw[0][c];
+                        bhi[c] = bbw[0][3 + c];
+                    }
+#pragma unroll
+                    for (int ww = 1; ww < NW; ++ww)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            blo[c] = fminf(blo[c], bbw[ww][c]);
+                            bhi[c] = fmaxf(bhi[c], bbw[ww][3 + c]);
+                        }
+                    f = box_gap2(blo, bhi, tlo, thi) >= T_FAR64 ? 1 : 0;
+                }
+                if (k == 0) tfar = f;
             }
             if (!first)  // the row's sums so far, loaded beside the tile
                 for (int e = (NW > 1 ? (int)threadIdx.x - 64 : (int)threadIdx.x); e >= 0 && e < NCOMP_S * 64;
@@ -1349,6 +1420,7 @@ __global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ 
                     if (j < cnt) prev[m][j] = jrow[(size_t)m * len + (size_t)(s0 - S0 + j)];
                 }
             __syncthreads();
+            const bool farz = __builtin_amdgcn_readfirstlane(tfar) != 0;
             bool live[TPT], diag[TPT], full = cnt == 64, any = false;
 #pragma unroll
             for (int p = 0; p < TPT; ++p) {
@@ -1363,7 +1435,10 @@ __global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ 
             if (any) {
 #pragma unroll 1
                 for (int sb = 0; sb < 64; sb += SB) {
-                    if (full) {
+                    if (full && farz) {
+                        sym64_steps<KIND, true, false, TPT, SB, false>(tile, sb, lane, me, live, nt,
+                                                                       diag, cnt, a, jacc);
+                    } else if (full) {
                         if (uniform)
                             sym64_steps<KIND, true, false, TPT, SB>(tile, sb, lane, me, live, nt,
                                                                     diag, cnt, a, jacc);

@@ -1,0 +1,135 @@
+"""The far-field form of the pair kernel against the CPU oracle.
+
+Pairs with t = rho^2/sigma^2 >= 1024 (binary64: decided per pair of 64-
+particle groups by their bounding boxes; in the symmetric option per target
+block and 64-source tile) and far tiles with t >= 128 (the binary32 symmetric
+pass) take the kernel values from one reciprocal square
+root and a short polynomial, with Zs and the PSE terms dropped (below binary64
+rounding there).  The oracle evaluates the reference's full kernel for every
+pair (highorder.h:20-60, gaussian.h:20-58, interaction_utils.h:429-481).
+Checked here: a particle set in which most of every velocity comes from far
+pairs (both kernels, both precisions, both binary64 schemes), pairs of clusters whose boxes sit just
+below, at and above the binary64 threshold, and that the far choice leaves the
+one-sided variants bitwise equal.  Bars: binary64 1e-10 per particle (the
+parity bar; the far form is measured at rounding level, printed); binary32
+1e-6 relative RMS."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import pawan_amd as pa
+from pawan_amd import generators as gen
+from helpers import per_particle_rel, rel_rms, sparse_lattice
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+TOL32 = 1e-6
+KIND = {pa.KERNEL_HIGHORDER: orc.HIGHORDER, pa.KERNEL_GAUSSIAN: orc.GAUSSIAN}
+
+
+def _rates(w, nu, precision, kernel=pa.KERNEL_HIGHORDER, **kw):
+    S = pa.Interaction(pa.Wake.from_dict(w), nu=nu, precision=precision, kernel=kernel, **kw)
+    S.solve()
+    r = S.getRates()
+    S.close()
+    n = w["n"]
+    return r[:3 * n].reshape(n, 3), r[3 * n:].reshape(n, 3)
+
+
+@pytest.mark.parametrize("kernel", [pa.KERNEL_HIGHORDER, pa.KERNEL_GAUSSIAN])
+def test_far_sparse_lattice_both_precisions(kernel):
+    """4,096 particles on a sparse lattice (sub-cubes of 64, 40 core radii
+    between neighbouring sub-cubes' boxes): every pair between two sub-cubes
+    is far, and those pairs carry most of every velocity."""
+    w = sparse_lattice()
+    nu = 1e-3
+    print(f"case far lattice kernel={kernel} N={w['n']}", flush=True)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL,
+                       kind=KIND[kernel])
+    own = np.zeros_like(rv)
+    for c in range(w["n"] // 64):
+        s = slice(64 * c, 64 * c + 64)
+        own[s] = orc.rates(w["pos"][s], w["vort"][s], w["radius"][s], w["volume"][s], nu,
+                           mode=orc.SERIAL, kind=KIND[kernel])[0]
+    share = np.linalg.norm(rv - own) / np.linalg.norm(rv)
+    v64, d64 = _rates(w, nu, pa.FP64, kernel)
+    e64v, e64d = per_particle_rel(v64, rv), per_particle_rel(d64, rd)
+    vs, ds = _rates(w, nu, pa.FP64, kernel, scheme=pa.SCHEME_SYMMETRIC)
+    esv, esd = per_particle_rel(vs, rv), per_particle_rel(ds, rd)
+    v32, d32 = _rates(w, nu, pa.FP32C, kernel)
+    e32v, e32d = rel_rms(v32, rv), rel_rms(d32, rd)
+    print(f"  far pairs' share of the velocity {share:.2f}; binary64 per particle: velocity "
+          f"{e64v:.2e} vorticity rate {e64d:.2e} (symmetric {esv:.2e} {esd:.2e}); binary32 "
+          f"rel RMS: {e32v:.2e} {e32d:.2e}", flush=True)
+    assert share > 0.5
+    assert e64v < TOL64 and e64d < TOL64, (e64v, e64d)
+    assert esv < TOL64 and esd < TOL64, (esv, esd)
+    assert e32v < TOL32 and e32d < TOL32, (e32v, e32d)
+
+
+def _two_clusters(gap, core=0.05, seed=11):
+    """Two 64-particle clusters of side 0.5 sigma whose bounding boxes are
+    `gap` core radii apart along x (sigma = core for a uniform core).  The
+    second carries no vorticity: its velocities come from the first cluster
+    alone, i.e. from the pairs between the two clusters."""
+    u = gen.uniform(seed, 12 * 64).reshape(128, 6)
+    side = 0.5 * core
+    a = side * u[:64, :3]
+    b = side * u[64:, :3]
+    # b's box starts `gap` sigma past a's box along x
+    b[:, 0] += a[:, 0].max() - b[:, 0].min() + gap * core
+    pos = np.concatenate([a, b])
+    vort = 0.01 * (u[:, 3:6] - 0.5)
+    vort[64:] = 0.0
+    n = 128
+    return {"pos": np.ascontiguousarray(pos), "vort": np.ascontiguousarray(vort),
+            "radius": np.full(n, core), "volume": np.full(n, 4.0 / 3.0 * np.pi * core ** 3),
+            "n": n}
+
+
+@pytest.mark.parametrize("gap", [20.0, 31.9, 32.0, 32.1, 45.0, 1000.0])
+def test_far_threshold_two_clusters(gap):
+    """Clusters around the binary64 threshold (box distance 32 sigma, t = 1024)
+    and past the binary32 one (11.3 sigma): the near and far forms agree with
+    the oracle on either side of it."""
+    w = _two_clusters(gap)
+    nu = 1e-3
+    print(f"case far threshold gap={gap} sigma", flush=True)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL)
+    v64, d64 = _rates(w, nu, pa.FP64)
+    v32, d32 = _rates(w, nu, pa.FP32C)
+    vs, ds = _rates(w, nu, pa.FP64, scheme=pa.SCHEME_SYMMETRIC)
+    esv, esd = per_particle_rel(vs, rv), per_particle_rel(ds, rd)
+    # the second cluster's velocities: all from pairs between the clusters
+    eb = per_particle_rel(v64[64:], rv[64:])
+    eb32 = rel_rms(v32[64:], rv[64:])
+    e64v, e64d = per_particle_rel(v64, rv), per_particle_rel(d64, rd)
+    e32v, e32d = rel_rms(v32, rv), rel_rms(d32, rd)
+    print(f"  binary64 per particle: velocity {e64v:.2e} (second cluster {eb:.2e}) "
+          f"vorticity rate {e64d:.2e}; binary32 rel RMS {e32v:.2e} (second cluster "
+          f"{eb32:.2e}) {e32d:.2e}; binary64 symmetric {esv:.2e} {esd:.2e}", flush=True)
+    assert e64v < TOL64 and e64d < TOL64 and eb < TOL64
+    assert esv < TOL64 and esd < TOL64
+    assert e32v < TOL32 and e32d < TOL32 and eb32 < TOL32
+
+
+def test_far_onesided_variants_bitwise_identical():
+    """The far choice is made per pair of 64-particle groups of the job, so
+    every one-sided binary64 variant (block, targets per thread, unroll) gives
+    the same bits on a set where most pairs are far."""
+    w = sparse_lattice()
+    print(f"case far variants N={w['n']}", flush=True)
+    ref = None
+    for block in (64, 128, 256):
+        for tpt in (1, 2):
+            for unroll in (1, 2, 4):
+                S = pa.Interaction(pa.Wake.from_dict(w), nu=1e-3, precision=pa.FP64, block=block,
+                                   chunk=1024, targets_per_thread=tpt, unroll=unroll,
+                                   scheme=pa.SCHEME_ONESIDED)
+                S.solve()
+                r = S.getRates()
+                S.close()
+                if ref is None:
+                    ref = r
+                assert np.array_equal(r, ref), (block, tpt, unroll)

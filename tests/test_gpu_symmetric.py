@@ -13,7 +13,7 @@ import pytest
 import oracle as orc
 import pawan_amd as pa
 from pawan_amd import generators as gen
-from helpers import per_particle_rel, rel_rms
+from helpers import per_particle_rel, rel_rms, sparse_lattice
 
 pytestmark = pytest.mark.gpu
 
@@ -189,31 +189,13 @@ def test_symmetric64_rk4_batch_alone_equals_mixed():
     assert ep < 1e-11 and ev < 1e-9
 
 
-def _sparse_lattice(sub=4, cubes=4, spacing=10.0, core=0.2, seed=7):
-    """Particles `spacing` apart on a (sub x cubes)^3 lattice, ordered sub-cube
-    by sub-cube (sub^3 = 64 particles: one tile each), jittered by 10%;
-    vorticity mostly along z.  Every distance appears as often as its
-    influence decays, so the pairs between different sub-cubes carry most of
-    every velocity."""
-    g = np.stack(np.meshgrid(*[np.arange(sub)] * 3, indexing="ij"), -1).reshape(-1, 3)
-    c = np.stack(np.meshgrid(*[np.arange(cubes)] * 3, indexing="ij"), -1).reshape(-1, 3)
-    lat = (c[:, None, :] * sub + g[None, :, :]).reshape(-1, 3).astype(float)
-    n = lat.shape[0]
-    u = gen.uniform(seed, 6 * n).reshape(n, 6)
-    pos = (lat - (sub * cubes - 1) / 2.0 + 0.1 * (2.0 * u[:, :3] - 1.0)) * spacing
-    vort = 0.01 * np.concatenate([0.2 * (u[:, 3:5] - 0.5), 0.5 + u[:, 5:6]], -1)
-    return {"pos": np.ascontiguousarray(pos), "vort": np.ascontiguousarray(vort),
-            "radius": np.full(n, core), "volume": np.full(n, 4.0 / 3.0 * np.pi * core ** 3),
-            "n": n}
-
-
 @pytest.mark.parametrize("block", [256, 512])
 def test_far_tiles_sparse_lattice(block):
     """A sparse lattice ordered by 64-particle sub-cubes: most sub-cubes are
     farther from a target block than 1/8 of the coordinates, so their tiles
     run wholly in binary32, and they carry most of every velocity; the rates
     still meet the binary32 bar against the oracle."""
-    w = _sparse_lattice()
+    w = sparse_lattice()
     nu = 1e-3
     print(f"case far tiles: sparse lattice N={w['n']} block={block}", flush=True)
     vel, dv = _rates(w, nu, pa.FP32C, block=block, targets_per_thread=1)
