@@ -81,14 +81,15 @@ LONG_DT = 1e-5
 
 # The variants the autotuner picks for the N = 262,144 rotor wake on MI355X
 # (used with --no-tune, e.g. under a profiler): binary64 one-sided 256 x 2
-# targets, unroll 2, 32 source chunks (116.0 against 117.2 ms per RHS at the
-# policy's 16); binary64 symmetric 256 x 1, 256 chunks and strips (97.5
-# against 98.1 at 128); compensated binary32 symmetric 256 x 1, 256 chunks
-# (55.8 against 56.7 at 128, 58.6 for 512 x 1).
+# targets, unroll 2, 32 source chunks (105.4 against 107.2 ms per RHS at the
+# policy's 16); binary64 symmetric 64 x 2 (blocks of 128 targets, whose
+# smaller boxes make more source tiles far), 512 chunks and strips (80.0
+# against 82.2 at the policy's, 86.3 for 256 x 1); compensated binary32
+# symmetric 256 x 1, 256 chunks (46.4 against 48.4 at 128, 49.0 for 512 x 1).
 MEASURED_BEST = {
     "fp64": {"block": 256, "targets_per_thread": 2, "unroll": 2, "scheme": 1, "chunks": 32},
-    "fp64_symmetric": {"block": 256, "targets_per_thread": 1, "unroll": 0, "scheme": 2,
-                       "chunks": 256},
+    "fp64_symmetric": {"block": 64, "targets_per_thread": 2, "unroll": 0, "scheme": 2,
+                       "chunks": 512},
     "fp32c": {"block": 256, "targets_per_thread": 1, "unroll": 0, "scheme": 2, "chunks": 256},
 }
 
