@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "vpm_types.hpp"
 
 namespace pw {
@@ -301,13 +303,17 @@ struct Acc32 {
 };
 
 // One pair in binary32.  Formed in binary64 and rounded: the displacement,
-// and the stretching cross product a_i x a_j (nearly parallel strengths along
-// a filament cancel in it; from binary32 strengths the error would be
-// eps32 |a|^2 per pair, which the cancelling sums over sources then expose).
+// the stretching cross product a_i x a_j (nearly parallel strengths along a
+// filament cancel in it; from binary32 strengths the error would be
+// eps32 |a|^2 per pair, which the cancelling sums over sources then expose),
+// and the PSE differences a_j - a_i, v_j - v_i (the epilogue's
+// v_i S_a - a_i S_v is the same either way; the differences keep the binary32
+// rounding of a_j and v_j out of the cancellation of the two sums in a smooth
+// core).  vi: the target's volume.
 template <int KIND, bool UNIFORM>
 __device__ __forceinline__ void pair32(double xi, double yi, double zi, double aix, double aiy,
-                                       double aiz, float hi, const Rec32 &s, float inv_s2,
-                                       Acc32 &a) {
+                                       double aiz, double vi, float hi, const Rec32 &s,
+                                       float inv_s2, Acc32 &a) {
     const float dx = (float)(xi - s.x), dy = (float)(yi - s.y), dz = (float)(zi - s.z);
     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
     float Q, Fp, Zs;
@@ -336,10 +342,35 @@ __device__ __forceinline__ void pair32(double xi, double yi, double zi, double a
     a.v[3] = __builtin_fmaf(fd, dx, __builtin_fmaf(Q, ex, a.v[3]));
     a.v[4] = __builtin_fmaf(fd, dy, __builtin_fmaf(Q, ey, a.v[4]));
     a.v[5] = __builtin_fmaf(fd, dz, __builtin_fmaf(Q, ez, a.v[5]));
-    a.v[6] = __builtin_fmaf(Zs, s.fx, a.v[6]);
-    a.v[7] = __builtin_fmaf(Zs, s.fy, a.v[7]);
-    a.v[8] = __builtin_fmaf(Zs, s.fz, a.v[8]);
-    a.v[9] = __builtin_fmaf(Zs, s.v, a.v[9]);
+    a.v[6] = __builtin_fmaf(Zs, (float)(s.ax - aix), a.v[6]);
+    a.v[7] = __builtin_fmaf(Zs, (float)(s.ay - aiy), a.v[7]);
+    a.v[8] = __builtin_fmaf(Zs, (float)(s.az - aiz), a.v[8]);
+    a.v[9] = __builtin_fmaf(Zs, (float)(s.vd - vi), a.v[9]);
+}
+
+// One far pair (t >= T_FAR32) of a uniform-core job in binary32: the far-
+// field kernel and no PSE terms; the displacement and the stretching cross
+// product as pair32 forms them.
+template <int KIND>
+__device__ __forceinline__ void pair32_far(double xi, double yi, double zi, double aix, double aiy,
+                                           double aiz, const Rec32 &s, Acc32 &a) {
+    const float dx = (float)(xi - s.x), dy = (float)(yi - s.y), dz = (float)(zi - s.z);
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    float Q, Fp;
+    kernel32_far<KIND>(__builtin_amdgcn_rsqf(r2), Q, Fp);
+    const float cx = __builtin_fmaf(s.fy, dz, -s.fz * dy);
+    const float cy = __builtin_fmaf(s.fz, dx, -s.fx * dz);
+    const float cz = __builtin_fmaf(s.fx, dy, -s.fy * dx);
+    a.v[0] = __builtin_fmaf(Q, cx, a.v[0]);
+    a.v[1] = __builtin_fmaf(Q, cy, a.v[1]);
+    a.v[2] = __builtin_fmaf(Q, cz, a.v[2]);
+    const float ex = (float)__builtin_fma(aiy, s.az, -aiz * s.ay);
+    const float ey = (float)__builtin_fma(aiz, s.ax, -aix * s.az);
+    const float ez = (float)__builtin_fma(aix, s.ay, -aiy * s.ax);
+    const float fd = Fp * __builtin_fmaf(dz, ez, __builtin_fmaf(dy, ey, dx * ex));
+    a.v[3] = __builtin_fmaf(fd, dx, __builtin_fmaf(Q, ex, a.v[3]));
+    a.v[4] = __builtin_fmaf(fd, dy, __builtin_fmaf(Q, ey, a.v[4]));
+    a.v[5] = __builtin_fmaf(fd, dz, __builtin_fmaf(Q, ez, a.v[5]));
 }
 
 // Knuth's TwoSum: s + e == a + b exactly (round-to-nearest, no reassociation).
@@ -519,10 +550,52 @@ __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ j
     }
 }
 
+// One 32-source sub-block [j0, j0 + 32) of a staged binary32 tile (first
+// source s0) against the thread's TPT targets of a uniform-core job; target
+// slot p takes it in the far form when bit p of FM is set.  DIAG: skip j == i.
+template <int KIND, int TPT, int UNR, unsigned FM, bool DIAG>
+__device__ __forceinline__ void sub32(const Rec32 *tile, int j0, const Rec32 *me, const float *hi,
+                                      float inv_s2, Acc32 *a, int s0, const int *idx) {
+#pragma unroll UNR
+    for (int j = j0; j < j0 + 32; ++j) {
+        const Rec32 sj = tile[j];
+#pragma unroll
+        for (int p = 0; p < TPT; ++p) {
+            if ((FM >> p) & 1u) {
+                pair32_far<KIND>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay, me[p].az, sj, a[p]);
+            } else {
+                if (DIAG && s0 + j == idx[p]) continue;
+                pair32<KIND, true>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay, me[p].az, me[p].vd,
+                                   hi[p], sj, inv_s2, a[p]);
+            }
+        }
+    }
+}
+
+template <int KIND, int TPT, int UNR, bool DIAG>
+__device__ __forceinline__ void sub32_fm(unsigned fm, const Rec32 *tile, int j0, const Rec32 *me,
+                                         const float *hi, float inv_s2, Acc32 *a, int s0,
+                                         const int *idx) {
+    constexpr unsigned ALL = (1u << TPT) - 1;
+    if (fm == 0)
+        sub32<KIND, TPT, UNR, 0, DIAG>(tile, j0, me, hi, inv_s2, a, s0, idx);
+    else if (fm == ALL)
+        sub32<KIND, TPT, UNR, ALL, DIAG>(tile, j0, me, hi, inv_s2, a, s0, idx);
+    else if constexpr (TPT == 2) {
+        if (fm == 1)
+            sub32<KIND, TPT, UNR, 1, DIAG>(tile, j0, me, hi, inv_s2, a, s0, idx);
+        else
+            sub32<KIND, TPT, UNR, 2, DIAG>(tile, j0, me, hi, inv_s2, a, s0, idx);
+    }
+}
+
 // Compensated binary32 pair pass: plain binary32 sums over SB sources, folded
 // into a (sum, carry) pair per component with TwoSum; the item's result is
 // sum + carry in binary64, and chunks are added in binary64 afterwards.  Sub-
 // block boundaries are fixed by the chunk start, whatever TB, TPT and UNR are.
+// A uniform-core job's pairs between a 64-target group and a 64-source group
+// whose boxes are at least sqrt(T_FAR32) apart take the far form (chosen per
+// aligned group pair, so the same for every variant).
 template <int KIND, int TB, int TPT, int UNR>
 __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ jobs,
                                                     const Item *__restrict__ items,
@@ -531,9 +604,10 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
     constexpr int ST = TB;
     constexpr int SB = 32;
     constexpr int TW = TB * TPT;
-    static_assert(ST % SB == 0, "sub-block must divide the LDS tile");
+    static_assert(ST % 64 == 0, "64-source groups must divide the LDS tile");
     __shared__ Rec32 tile[ST];
     __shared__ int slot;
+    const int lane = (int)threadIdx.x & 63;
     const int nitems = *nitems_p;
     int taken = 0;
     for (;;) {
@@ -548,13 +622,15 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
         Rec32 me[TPT];
         float hi[TPT];
         Acc32 sum[TPT], car[TPT];
+        float tlo[TPT][3], thi[TPT][3];  // box of this wave's 64 targets of slot p
 #pragma unroll
         for (int p = 0; p < TPT; ++p) {
             idx[p] = t0 + p * TB + (int)threadIdx.x;
-            me[p] = rec[idx[p] < n ? idx[p] : n - 1];
+            me[p] = rec[idx[p] < n ? idx[p] : n - 1];  // targets past n repeat particle n - 1
             hi[p] = (float)me[p].h;
 #pragma unroll
             for (int m = 0; m < NCOMP; ++m) sum[p].v[m] = car[p].v[m] = 0.0f;
+            wave_box(me[p].x, me[p].y, me[p].z, tlo[p], thi[p]);
         }
         const float inv_s2 = (float)J.inv_s2;
         const bool uniform = J.uniform != 0;
@@ -570,32 +646,39 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
             }
             __syncthreads();
             const bool diag = (s0 < t0 + TW) && (t0 < s0 + cnt);
+            unsigned fm = 0;  // far mask of the 64-source group holding the sub-block
             for (int b0 = 0; b0 < cnt; b0 += SB) {
                 const int bn = min(SB, cnt - b0);
+                if ((b0 & 63) == 0) {
+                    fm = 0;
+                    if (uniform && b0 + 64 <= cnt) {
+                        const Rec32 &q = tile[b0 + lane];
+                        float slo[3], shi[3];
+                        wave_box(q.x, q.y, q.z, slo, shi);
+#pragma unroll
+                        for (int p = 0; p < TPT; ++p)
+                            if (box_gap2(tlo[p], thi[p], slo, shi) >= T_FAR32) fm |= 1u << p;
+                        fm = __builtin_amdgcn_readfirstlane(fm);
+                    }
+                }
                 Acc32 a[TPT];
 #pragma unroll
                 for (int p = 0; p < TPT; ++p)
 #pragma unroll
                     for (int m = 0; m < NCOMP; ++m) a[p].v[m] = 0.0f;
-                if (!diag && bn == SB) {
-                    if (uniform) {
+                if (uniform && bn == SB) {
+                    if (diag)
+                        sub32_fm<KIND, TPT, UNR, true>(fm, tile, b0, me, hi, inv_s2, a, s0, idx);
+                    else
+                        sub32_fm<KIND, TPT, UNR, false>(fm, tile, b0, me, hi, inv_s2, a, s0, idx);
+                } else if (!diag && bn == SB) {
 #pragma unroll UNR
-                        for (int j = 0; j < SB; ++j) {
-                            const Rec32 sj = tile[b0 + j];
+                    for (int j = 0; j < SB; ++j) {
+                        const Rec32 sj = tile[b0 + j];
 #pragma unroll
-                            for (int p = 0; p < TPT; ++p)
-                                pair32<KIND, true>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
-                                                   me[p].az, hi[p], sj, inv_s2, a[p]);
-                        }
-                    } else {
-#pragma unroll UNR
-                        for (int j = 0; j < SB; ++j) {
-                            const Rec32 sj = tile[b0 + j];
-#pragma unroll
-                            for (int p = 0; p < TPT; ++p)
-                                pair32<KIND, false>(me[p].x, me[p].y, me[p].z, me[p].ax,
-                                                    me[p].ay, me[p].az, hi[p], sj, inv_s2, a[p]);
-                        }
+                        for (int p = 0; p < TPT; ++p)
+                            pair32<KIND, false>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
+                                                me[p].az, me[p].vd, hi[p], sj, inv_s2, a[p]);
                     }
                 } else {
                     for (int j = 0; j < bn; ++j) {
@@ -605,10 +688,11 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
                             if (s0 + b0 + j == idx[p]) continue;
                             if (uniform)
                                 pair32<KIND, true>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay,
-                                                   me[p].az, hi[p], sj, inv_s2, a[p]);
+                                                   me[p].az, me[p].vd, hi[p], sj, inv_s2, a[p]);
                             else
                                 pair32<KIND, false>(me[p].x, me[p].y, me[p].z, me[p].ax,
-                                                    me[p].ay, me[p].az, hi[p], sj, inv_s2, a[p]);
+                                                    me[p].ay, me[p].az, me[p].vd, hi[p], sj, inv_s2,
+                                                    a[p]);
                         }
                     }
                 }
@@ -683,12 +767,14 @@ struct SrcTile {
     float4 f[128];                      // (fx, fy, fz, v)
     float4 p[128];                      // (x, y, z, h) rounded to binary32: far tiles
     float h[128];
+    double vd[128];                     // binary64 volumes: PSE differences of close pairs
 };
 
 // Fields of one particle in registers.
 struct Part32 {
     double x, y, z, ax, ay, az;
     float fx, fy, fz, v, h;
+    double vd;
 };
 // The same for a far tile: every field in binary32.
 struct Part32f {
@@ -747,6 +833,7 @@ __device__ __forceinline__ Part32 load_src(const SrcTile &t, int k, Part32) {
     p.fz = f.z;
     p.v = f.w;
     p.h = t.h[k];
+    p.vd = t.vd[k];
     return p;
 }
 
@@ -802,12 +889,28 @@ __device__ __forceinline__ void pair32s(const PT &me, const PT &s, bool valid, A
     a.v[3] += sx;
     a.v[4] += sy;
     a.v[5] += sz;
-    // PSE sums: i gets k a_j, k v_j; j gets k a_i, k v_i
+    // PSE sums.  A far tile's pair (Part32f): i gets k a_j, k v_j and j gets
+    // k a_i, k v_i.  A close pair (Part32) adds the differences, formed in
+    // binary64 and rounded: i gets k (a_j - a_i), k (v_j - v_i), j their
+    // negatives.  The epilogue's v_i S_a - a_i S_v is the same sum either way,
+    // k (v_i a_j - v_j a_i) per pair; in a smooth core the two sums agree to a
+    // digit or two, and the difference form keeps the rounding of a_j and v_j
+    // to binary32 out of that cancellation.
+    float tg[4], sg[4], zsg;
+    if constexpr (std::is_same<PT, Part32>::value) {
+        tg[0] = sg[0] = (float)(s.ax - me.ax);
+        tg[1] = sg[1] = (float)(s.ay - me.ay);
+        tg[2] = sg[2] = (float)(s.az - me.az);
+        tg[3] = sg[3] = (float)(s.vd - me.vd);
+        zsg = -Zs;
+    } else {
+        tg[0] = s.fx, tg[1] = s.fy, tg[2] = s.fz, tg[3] = s.v;
+        sg[0] = me.fx, sg[1] = me.fy, sg[2] = me.fz, sg[3] = me.v;
+        zsg = Zs;
+    }
     if (DIFF) {
-        a.v[6] = __builtin_fmaf(Zs, s.fx, a.v[6]);
-        a.v[7] = __builtin_fmaf(Zs, s.fy, a.v[7]);
-        a.v[8] = __builtin_fmaf(Zs, s.fz, a.v[8]);
-        a.v[9] = __builtin_fmaf(Zs, s.v, a.v[9]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) a.v[6 + m] = __builtin_fmaf(Zs, tg[m], a.v[6 + m]);
     }
     if (FIRST) {
         b.v[0] = __builtin_fmaf(uy, me.fz, -uz * me.fy);
@@ -817,10 +920,8 @@ __device__ __forceinline__ void pair32s(const PT &me, const PT &s, bool valid, A
         b.v[4] = sy;
         b.v[5] = sz;
         if (DIFF) {
-            b.v[6] = Zs * me.fx;
-            b.v[7] = Zs * me.fy;
-            b.v[8] = Zs * me.fz;
-            b.v[9] = Zs * me.v;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) b.v[6 + m] = zsg * sg[m];
         }
     } else {
         b.v[0] = __builtin_fmaf(uy, me.fz, __builtin_fmaf(-uz, me.fy, b.v[0]));
@@ -830,10 +931,8 @@ __device__ __forceinline__ void pair32s(const PT &me, const PT &s, bool valid, A
         b.v[4] += sy;
         b.v[5] += sz;
         if (DIFF) {
-            b.v[6] = __builtin_fmaf(Zs, me.fx, b.v[6]);
-            b.v[7] = __builtin_fmaf(Zs, me.fy, b.v[7]);
-            b.v[8] = __builtin_fmaf(Zs, me.fz, b.v[8]);
-            b.v[9] = __builtin_fmaf(Zs, me.v, b.v[9]);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) b.v[6 + m] = __builtin_fmaf(zsg, sg[m], b.v[6 + m]);
         }
     }
 }
@@ -953,7 +1052,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
             wt[p] = (B0 + p * TB) / 64 + w;  // this wave's 64-target tile
             nt[p] = max(0, min(64, n - wt[p] * 64));
             const Rec32 r = rec[idx[p] < n ? idx[p] : n - 1];
-            me[p] = Part32{r.x, r.y, r.z, r.ax, r.ay, r.az, r.fx, r.fy, r.fz, r.v, (float)r.h};
+            me[p] = Part32{r.x, r.y, r.z, r.ax, r.ay, r.az, r.fx, r.fy, r.fz, r.v, (float)r.h, r.vd};
 #pragma unroll
             for (int m = 0; m < NCOMP_S; ++m) isum[p][m] = 0.0;
         }
@@ -996,7 +1095,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                 if (k < cnt) {
                     r = rec[s0 + k];
                 } else {
-                    r.x = r.y = r.z = r.h = r.ax = r.ay = r.az = 0.0;
+                    r.x = r.y = r.z = r.h = r.ax = r.ay = r.az = r.vd = 0.0;
                     r.fx = r.fy = r.fz = r.v = 0.0f;
                 }
                 const float px = (float)r.x, py = (float)r.y, pz = (float)r.z;
@@ -1008,6 +1107,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                     tile.f[k + c] = make_float4(r.fx, r.fy, r.fz, r.v);
                     tile.p[k + c] = make_float4(px, py, pz, (float)r.h);
                     tile.h[k + c] = (float)r.h;
+                    tile.vd[k + c] = r.vd;
                 }
                 // far test (full tiles only): the tile's box against the
                 // block's (the waves' boxes were written before the barrier

@@ -120,7 +120,7 @@ static void fill_records(void *dst, int precision, size_t m, const double *pos, 
             r.fy = (float)vort[3 * i + 1];
             r.fz = (float)vort[3 * i + 2];
             r.v = (float)volume[i];
-            r.pad[0] = r.pad[1] = 0.0f;
+            r.vd = volume[i];
         }
     }
 }

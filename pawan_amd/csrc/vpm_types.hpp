@@ -63,13 +63,15 @@ struct __align__(16) Rec64 {
 // Record of the compensated binary32 path: binary64 positions (the
 // displacement is formed in binary64 and then rounded) and binary64
 // strengths (the stretching cross product a_i x a_j is formed in binary64:
-// nearly parallel strengths would lose it in binary32), plus a binary32 copy
-// of the strengths for the velocity and diffusion terms.  80 bytes, 5 x 16 B.
+// nearly parallel strengths would lose it in binary32), a binary32 copy of
+// the strengths and the volume for the velocity and diffusion terms, and the
+// volume in binary64 (close pairs form the PSE differences v_j - v_i in
+// binary64).  80 bytes, 5 x 16 B.
 struct __align__(16) Rec32 {
     double x, y, z, h;
     double ax, ay, az;
     float fx, fy, fz, v;
-    float pad[2];
+    double vd;
 };
 
 static_assert(sizeof(Rec64) == 64, "Rec64 is four 16-byte vectors");
