@@ -276,13 +276,18 @@ __device__ __forceinline__ float uniform_f(float v) {
 }
 
 // Bounding box of the 64 points (x, y, z) held one per lane, in binary32, in
-// wave-uniform registers: lo[0..2], hi[0..2].
+// wave-uniform registers: lo[0..2], hi[0..2].  The box holds the binary64
+// points: rounding a coordinate to binary32 moves it by at most 2^-24 of its
+// magnitude, and the box is widened by 2^-22 of its corners' (at 2^30 sigma
+// from the origin a rounded box could otherwise open a 128-sigma gap between
+// touching groups).
 __device__ __forceinline__ void wave_box(double x, double y, double z, float *lo, float *hi) {
     const float q[3] = {(float)x, (float)y, (float)z};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        lo[c] = uniform_f(wave_min(q[c]));
-        hi[c] = uniform_f(wave_max(q[c]));
+        const float l = uniform_f(wave_min(q[c])), h = uniform_f(wave_max(q[c]));
+        lo[c] = __builtin_fmaf(-0x1p-22f, fabsf(l), l);
+        hi[c] = __builtin_fmaf(0x1p-22f, fabsf(h), h);
     }
 }
 
@@ -1401,8 +1406,10 @@ __device__ __forceinline__ void sym64_steps(const SrcTile64 &tile, int s_begin, 
     }
 }
 
+// Held to two waves per SIMD (at most 256 VGPRs): the 64 x 2 variant the tuner
+// picks at N = 262,144 otherwise takes 258 and runs one wave per SIMD.
 template <int KIND, int TB, int TPT>
-__global__ __launch_bounds__(TB) void pair64s_kernel(const JobDev *__restrict__ jobs,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(2))) void pair64s_kernel(const JobDev *__restrict__ jobs,
                                                      const Item *__restrict__ items,
                                                      const int *__restrict__ nitems_p,
                                                      int *__restrict__ counter) {

@@ -114,6 +114,34 @@ def test_far_threshold_two_clusters(gap):
     assert e32v < TOL32 and e32d < TOL32 and eb32 < TOL32
 
 
+def test_far_choice_far_from_origin():
+    """Two touching clusters (boxes 1 sigma apart, sigma = 1/16 so that the
+    records' coordinates in units of sigma are exact) placed about 2^30 sigma
+    from the origin, with the binary32 rounding boundary between the two
+    boxes: rounded to nearest, the boxes would read 128 sigma apart.  Widened
+    boxes keep these pairs in the full form, and every path matches the
+    oracle as near the origin."""
+    core = 1.0 / 16.0
+    w = _two_clusters(1.0, core=core)
+    w["pos"][:, 0] += (2.0 ** 30 + 62.75) * core
+    nu = 1e-3
+    lo_b = w["pos"][64:, 0].min() / core
+    print(f"case far choice far from the origin: second box from {lo_b:.4f} sigma", flush=True)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL)
+    out = []
+    for prec, sch in ((pa.FP64, pa.SCHEME_ONESIDED), (pa.FP64, pa.SCHEME_SYMMETRIC),
+                      (pa.FP32C, pa.SCHEME_ONESIDED), (pa.FP32C, pa.SCHEME_SYMMETRIC)):
+        v, d = _rates(w, nu, prec, scheme=sch)
+        if prec == pa.FP64:
+            e = (per_particle_rel(v, rv), per_particle_rel(d, rd))
+            assert e[0] < TOL64 and e[1] < TOL64, (prec, sch, e)
+        else:
+            e = (rel_rms(v, rv), rel_rms(d, rd))
+            assert e[0] < TOL32 and e[1] < TOL32, (prec, sch, e)
+        out.append(f"{prec}/{sch}: {e[0]:.2e} {e[1]:.2e}")
+    print("  " + "; ".join(out), flush=True)
+
+
 def test_far_onesided_variants_bitwise_identical():
     """The far choice is made per pair of 64-particle groups of the job, so
     every one-sided binary64 variant (block, targets per thread, unroll) gives
