@@ -1021,7 +1021,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
     __shared__ double prev[NCOMP_S][64];    // the strip row's sums before this block
     __shared__ float bbw[NW][6];            // per wave: box of its targets (lo xyz, hi xyz)
     __shared__ int tfar;                    // the staged tile is far from the block (1),
-                                            // and t >= T_FAR32 for all its pairs (2)
+                                            // and t >= T_FAR32 for all its pairs (2); not
+                                            // far, but t >= T_FAR32 (3)
     __shared__ int slot;
     const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int nitems = *nitems_p;
@@ -1061,26 +1062,17 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
             for (int m = 0; m < NCOMP_S; ++m) isum[p][m] = 0.0;
         }
-        {  // this wave's share of the block's box, in binary32 coordinates
-            float lo[3], hi[3];
+        {  // this wave's share of the block's box (targets past n repeat particle n - 1)
+            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                lo[c] = INFINITY;
-                hi[c] = -INFINITY;
-            }
-#pragma unroll
-            for (int p = 0; p < TPT; ++p) {  // targets past n repeat particle n - 1
-                const float q[3] = {(float)me[p].x, (float)me[p].y, (float)me[p].z};
+            for (int p = 0; p < TPT; ++p) {
+                float plo[3], phi[3];
+                wave_box(me[p].x, me[p].y, me[p].z, plo, phi);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    lo[c] = fminf(lo[c], q[c]);
-                    hi[c] = fmaxf(hi[c], q[c]);
+                    lo[c] = fminf(lo[c], plo[c]);
+                    hi[c] = fmaxf(hi[c], phi[c]);
                 }
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                lo[c] = wave_min(lo[c]);
-                hi[c] = wave_max(hi[c]);
             }
             if (lane == 0)
 #pragma unroll
@@ -1119,9 +1111,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                 // above)
                 int f = 0;
                 if (cnt == 64) {
-                    const float tlo[3] = {wave_min(px), wave_min(py), wave_min(pz)};
-                    const float thi[3] = {wave_max(px), wave_max(py), wave_max(pz)};
-                    float blo[3], bhi[3];
+                    float tlo[3], thi[3], blo[3], bhi[3];
+                    wave_box(r.x, r.y, r.z, tlo, thi);
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         blo[c] = bbw[0][c];
@@ -1137,7 +1128,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                     float d2;
                     f = far_boxes(blo, bhi, tlo, thi, d2) ? 1 : 0;
                     // uniform core: coordinates are in units of sigma, t = rho^2
-                    if (f && uniform && d2 >= T_FAR32) f = 2;
+                    if (uniform && d2 >= T_FAR32) f = f ? 2 : 3;
                 }
                 if (k == 0) tfar = f;
             }
@@ -1149,7 +1140,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                 }
             __syncthreads();
             const int farv = __builtin_amdgcn_readfirstlane(tfar);
-            const bool far = farv != 0, farz = farv == 2;
+            // far: binary32 positions and strengths; ffield: the far-field
+            // kernel and no PSE terms (binary64 displacement and cross product
+            // unless also far)
+            const bool far = farv == 1 || farv == 2, ffield = farv >= 2;
             bool live[TPT], diag[TPT], full = cnt == 64, any = false;
 #pragma unroll
             for (int p = 0; p < TPT; ++p) {
@@ -1181,7 +1175,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                             for (int p = 0; p < TPT; ++p)
                                 mf[p] = Part32f{(float)me[p].x, (float)me[p].y, (float)me[p].z,
                                                 me[p].fx, me[p].fy, me[p].fz, me[p].v, me[p].h};
-                            if (farz)
+                            if (ffield)
                                 sym_steps<KIND, true, false, TPT, SBJ, false>(tile, sj, lane, mf, live,
                                                                               nt, diag, cnt, a, jacc);
                             else if (uniform)
@@ -1191,7 +1185,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                                 sym_steps<KIND, false, false, TPT, SBJ>(tile, sj, lane, mf, live,
                                                                         nt, diag, cnt, a, jacc);
                         } else if (full) {
-                            if (uniform)
+                            if (ffield)
+                                sym_steps<KIND, true, false, TPT, SBJ, false>(tile, sj, lane, me, live,
+                                                                              nt, diag, cnt, a, jacc);
+                            else if (uniform)
                                 sym_steps<KIND, true, false, TPT, SBJ>(tile, sj, lane, me, live, nt,
                                                                        diag, cnt, a, jacc);
                             else
@@ -1215,7 +1212,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                         // their order are those of the plain update: bitwise the same
                         // rates, 0.35% faster at N = 262,144)
                         for (int m = 0; m < NCOMP_S; ++m)
-                            if (m < 6 || !(full && farz))  // far-field runs add no PSE sums
+                            if (m < 6 || !(full && ffield))  // far-field runs add no PSE sums
                                 __hip_atomic_fetch_add(&jl[w][m][k], (double)jacc.v[m], __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_WAVEFRONT);
                         // the next run's lanes update entries this run's
