@@ -464,8 +464,9 @@ __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ j
     constexpr int ST = TB;
     constexpr int TW = TB * TPT;  // targets per tile
     __shared__ Rec64 tile[ST];
+    __shared__ float sbox[ST / 64][6];  // boxes of the staged tile's 64-source groups
     __shared__ int slot;
-    const int lane = (int)threadIdx.x & 63;
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int nitems = *nitems_p;
     int taken = 0;
     for (;;) {
@@ -500,6 +501,19 @@ __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ j
                 double2 *dst = (double2 *)tile;
                 for (int k = threadIdx.x; k < cnt * 4; k += TB) dst[k] = src[k];
             }
+            // wave w: the box of source group w, when full (read beside the
+            // copy, shared through LDS)
+            if (uniform && (w + 1) * 64 <= cnt) {
+                const Rec64 &q = rec[s0 + w * 64 + lane];
+                float lo[3], hi[3];
+                wave_box(q.x, q.y, q.z, lo, hi);
+                if (lane == 0)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        sbox[w][c] = lo[c];
+                        sbox[w][3 + c] = hi[c];
+                    }
+            }
             __syncthreads();
             // a tile holding some of the block's own targets skips j == i
             const bool diag = (s0 < t0 + TW) && (t0 < s0 + cnt);
@@ -511,9 +525,12 @@ __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ j
                     const int gn = min(64, cnt - g0);
                     unsigned fm = 0;
                     if (gn == 64) {
-                        const Rec64 &q = tile[g0 + lane];
                         float slo[3], shi[3];
-                        wave_box(q.x, q.y, q.z, slo, shi);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            slo[c] = uniform_f(sbox[g0 >> 6][c]);
+                            shi[c] = uniform_f(sbox[g0 >> 6][3 + c]);
+                        }
 #pragma unroll
                         for (int p = 0; p < TPT; ++p)
                             if (box_gap2(tlo[p], thi[p], slo, shi) >= T_FAR64) fm |= 1u << p;
