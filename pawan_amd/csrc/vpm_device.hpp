@@ -628,8 +628,9 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
     constexpr int TW = TB * TPT;
     static_assert(ST % 64 == 0, "64-source groups must divide the LDS tile");
     __shared__ Rec32 tile[ST];
+    __shared__ float sbox[ST / 64][6];  // boxes of the staged tile's 64-source groups
     __shared__ int slot;
-    const int lane = (int)threadIdx.x & 63;
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int nitems = *nitems_p;
     int taken = 0;
     for (;;) {
@@ -666,6 +667,19 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
                 float4 *dst = (float4 *)tile;
                 for (int k = threadIdx.x; k < cnt * 5; k += TB) dst[k] = src[k];
             }
+            // wave w: the box of source group w, when full (read beside the
+            // copy, shared through LDS)
+            if (uniform && (w + 1) * 64 <= cnt) {
+                const Rec32 &q = rec[s0 + w * 64 + lane];
+                float lo[3], hi[3];
+                wave_box(q.x, q.y, q.z, lo, hi);
+                if (lane == 0)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        sbox[w][c] = lo[c];
+                        sbox[w][3 + c] = hi[c];
+                    }
+            }
             __syncthreads();
             const bool diag = (s0 < t0 + TW) && (t0 < s0 + cnt);
             unsigned fm = 0;  // far mask of the 64-source group holding the sub-block
@@ -674,9 +688,12 @@ __global__ __launch_bounds__(TB) void pair32_kernel(const JobDev *__restrict__ j
                 if ((b0 & 63) == 0) {
                     fm = 0;
                     if (uniform && b0 + 64 <= cnt) {
-                        const Rec32 &q = tile[b0 + lane];
                         float slo[3], shi[3];
-                        wave_box(q.x, q.y, q.z, slo, shi);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            slo[c] = uniform_f(sbox[b0 >> 6][c]);
+                            shi[c] = uniform_f(sbox[b0 >> 6][3 + c]);
+                        }
 #pragma unroll
                         for (int p = 0; p < TPT; ++p)
                             if (box_gap2(tlo[p], thi[p], slo, shi) >= T_FAR32) fm |= 1u << p;
