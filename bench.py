@@ -44,24 +44,29 @@ FLOP_PER_PAIR = 95.0          # algorithmic FLOP per ordered pair (one-sided for
 FLOP_PER_PAIR_SYM = 55.0      # the symmetric form: ~110 per unordered pair, shared by both
 FP64_VECTOR_PEAK = 78.6e12    # MI355X FP64 vector FLOP/s
 FP32_VECTOR_PEAK = 157.3e12   # MI355X FP32 vector FLOP/s (published, packed)
-# HBM bytes fetched per binary64 pair-kernel launch at N = 262,144 with
-# 256-thread blocks, by (targets per thread, source chunks): rocprofv3
-# FETCH_SIZE in KiB per dispatch (mean over the dispatches), scaled by what
-# the same counter reads for a cold 2 GiB copy of known size (x2).  The kernel
-# writes exactly its partial sums, chunks x 10 x npad x 8 bytes (WRITE_SIZE,
-# x1: 336 MB at 16 chunks, 671 MB at 32); profiles/pair_kernels_rotor_wake_262144.md
-PAIR64_FETCH_KIB_262144 = {(2, 32): 319_307, (2, 16): 210_465,
-                           (1, 32): 325_718, (1, 16): 218_363}
+# HBM bytes per binary64 pair-kernel launch at N = 262,144 with 256-thread
+# blocks, by (targets per thread, source chunks) -- every chunk count the
+# autotuner tries (the policy's 16 and 1/2, 3/4, 3/2, 2x of it): rocprofv3
+# FETCH_SIZE and WRITE_SIZE in KiB per dispatch (mean over 7 dispatches), FETCH
+# scaled by what the same counter reads for a cold 2 GiB copy of known size
+# (x2), WRITE as read (x1; the kernel writes exactly its partial sums,
+# chunks x 10 x npad x 8 bytes); profiles/pair_kernels_rotor_wake_262144.md
+PAIR64_KIB_262144 = {  # (targets per thread, chunks): (FETCH_SIZE, WRITE_SIZE)
+    (1, 8): (166_713, 164_096), (1, 12): (177_171, 246_144), (1, 16): (204_471, 328_192),
+    (1, 24): (265_307, 492_288), (1, 32): (329_679, 656_384),
+    (2, 8): (160_770, 163_968), (2, 12): (184_528, 245_952), (2, 16): (208_962, 327_936),
+    (2, 24): (265_057, 491_904), (2, 32): (322_809, 655_872),
+}
 
 
 def pair64_traffic(n, block, targets_per_thread, chunks):
     """HBM bytes of one binary64 pair launch at N = 262,144 for a profiled
     variant (None otherwise)."""
     key = (targets_per_thread, chunks)
-    if n != 262144 or block != 256 or key not in PAIR64_FETCH_KIB_262144:
+    if n != 262144 or block != 256 or key not in PAIR64_KIB_262144:
         return None
-    npad = -(-n // 256) * 256
-    return 2.0 * PAIR64_FETCH_KIB_262144[key] * 1024.0 + chunks * 10 * npad * 8.0
+    fetch, write = PAIR64_KIB_262144[key]
+    return (2.0 * fetch + write) * 1024.0
 
 
 def log(*a):
