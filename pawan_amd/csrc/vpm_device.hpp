@@ -96,26 +96,25 @@ __device__ __forceinline__ void kernel64_scaled(double t, double R, double &Q, d
 //   HIGHORDER: (t+1.5)(t+1)^-2 = u g(u), u = 1/t, with
 //              g(u) = (1 + 1.5u)(1 + u)^-2 = sum_k (-1)^k (1 - k/2) u^k
 //              = 1 - u/2 + u^3/2 - u^4 + 3u^5/2 - 2u^6 + ...
-//              truncated after u^5 (the u^6 term is below 2e-18 relative);
-//              Zs = (1.875/pi)(t+1)^-7.5 is dropped: it changes Q and F' by
-//              less than 8 t^-6 relative (7e-18 at t = 1024) and its PSE
-//              terms are smaller than the pair's stretching terms by the
-//              same factor.
+//              truncated after u^4 (the u^5 term is below 1.4e-15 relative);
+//              Zs = (1.875/pi)(t+1)^-7.5 is dropped: it changes Q by less
+//              than 8 t^-6 relative (7e-18 at t = 1024) and F' by less than
+//              2.5 t^-5 (2.2e-15), and it is below 7e-15 of Q there.
 //   GAUSSIAN : erf(sqrt(t/2)) = 1 and exp(-t/2) = 0 in binary64 (t/2 >= 512),
 //              so Q = (0.25/pi) t^-1.5 and F' = -3 Q / t.
-// Checked in multiple precision over t in [1024, 1e13]: Q and F' within
-// 1e-15 and 2.2e-15 relative (the full form: 1.1e-15 and 1.4e-15).  A pair
+// Checked in multiple precision over t in [1024, 1e13] with the operation
+// order below: Q and F' within 2.0e-15 and 1.6e-15 relative (the full form
+// evaluated the same way: 1.2e-15 and 1.6e-15).  A pair
 // uses the far form when the bounding boxes of its target's 64-particle group
 // and its source's 64-particle group are at least sqrt(T_FAR64) sigma apart:
 // the choice depends on the particles alone, never on the kernel variant.
 constexpr float T_FAR64 = 1024.0f;
 // K_HC64-scaled coefficients of g in powers of it = 4/t (u = it/4): G_k =
-// 4 K_HC64 c_k / 4^k, c = 1, -1/2, 0, 1/2, -1, 3/2.
+// 4 K_HC64 c_k / 4^k, c = 1, -1/2, 0, 1/2, -1.
 constexpr double K_G0 = 4.0 * K_HC64;
 constexpr double K_G1 = -0.5 * K_HC64;
 constexpr double K_G3 = K_HC64 / 32.0;
 constexpr double K_G4 = -K_HC64 / 64.0;
-constexpr double K_G5 = 3.0 * K_HC64 / 512.0;
 
 // Binary32 kernel.  HIGHORDER is scaled by kz = kq = pi/1.875 (pair_scale):
 // the 1.875/pi of Zs moves into the epilogue.
@@ -221,9 +220,9 @@ __device__ __forceinline__ void kernel64_far(double r2, double &Q, double &itm) 
         const double R = rsq2_nr(r2);  // 2 t^-1/2
         const double it = R * R;       // 4 / t
         const double it2 = it * it;
-        // G0 + G1 it + it^3 (G3 + G4 it + G5 it^2)
-        const double g = __builtin_fma(__builtin_fma(__builtin_fma(K_G5, it, K_G4), it, K_G3) * it,
-                                       it2, __builtin_fma(K_G1, it, K_G0));
+        // G0 + G1 it + it^3 (G3 + G4 it)
+        const double g = __builtin_fma(__builtin_fma(K_G4, it, K_G3) * it, it2,
+                                       __builtin_fma(K_G1, it, K_G0));
         Q = (g * R) * it2;             // kq Q
         itm = -0.75 * it;              // F'/Q
     } else {
@@ -236,7 +235,7 @@ __device__ __forceinline__ void kernel64_far(double r2, double &Q, double &itm) 
 
 // One far pair (t >= T_FAR64) of a uniform-core job in binary64: the far-
 // field kernel, no PSE terms, and the stretching written as
-// q (e + (F'/Q)(d.e) d), one operation fewer.  43 arithmetic operations and
+// q (e + (F'/Q)(d.e) d), one operation fewer.  42 arithmetic operations and
 // one v_rsq_f64 against 54 and two for the full pair.
 template <int KIND>
 __device__ __forceinline__ void pair64_far(double xi, double yi, double zi, double aix, double aiy,
