@@ -235,7 +235,7 @@ __device__ __forceinline__ void kernel64_far(double r2, double &Q, double &itm) 
 
 // One far pair (t >= T_FAR64) of a uniform-core job in binary64: the far-
 // field kernel, no PSE terms, and the stretching written as
-// q (e + (F'/Q)(d.e) d), one operation fewer.  42 arithmetic operations and
+// q (e + (F'/Q)(d.e) d), one operation fewer.  43 arithmetic operations and
 // one v_rsq_f64 against 54 and two for the full pair.
 template <int KIND>
 __device__ __forceinline__ void pair64_far(double xi, double yi, double zi, double aix, double aiy,
