@@ -130,6 +130,31 @@ def run_path(pa, w, precision, steps, warmup, variant=None):
     return S, out
 
 
+def boundary_rate(S, w, reps=3):
+    """One RHS with the state handed over from host memory and the rates
+    handed back, as pawan's setStates / solve / getRates do (interaction.cpp:
+    204-229): the pair rate including both PCIe copies (never the bench's
+    value, which keeps the state resident)."""
+    n = w["n"]
+    x = np.concatenate([w["pos"].ravel(), w["vort"].ravel()])
+    r = np.empty(6 * n)
+    S.setStates(x)
+    S.solve()
+    S.getRates(r)
+    wall = dev = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        S.setStates(x)
+        S.solve()
+        S.getRates(r)
+        wall += time.perf_counter() - t0
+        dev += S.last_timing()["total_ms"]
+    ms = wall * 1e3 / reps
+    return {"ms_per_rhs": ms, "device_ms_per_rhs": dev / reps,
+            "pairs_per_s": n * (n - 1.0) / (ms * 1e-3), "state_bytes_each_way": 48 * n,
+            "note": "host state in, one RHS, host rates out (PCIe both ways); not the value"}
+
+
 def rates_split_error(S64, S32, w):
     """Relative RMS error of the binary32 path's rates vs the binary64 path at
     the initial state, per term: velocity, stretching (nu = 0) and PSE
@@ -763,6 +788,9 @@ def main():
         # steps and after the timed ones (outside the timed region)
         "drift": r64["drift"],
     }
+    line["boundary"] = boundary_rate(S64, w)
+    log(f"fp64 across the host boundary: {line['boundary']['ms_per_rhs']:.2f} ms per RHS "
+        f"({line['boundary']['device_ms_per_rhs']:.2f} on the device)")
     # each path's rates at the initial state, checked against the oracle in
     # the CPU leg
     gpu = {"fp64": gpu_terms(S64, w)}
