@@ -85,16 +85,16 @@ STEP_DT = 1e-5
 LONG_DT = 1e-5
 
 # The variants the autotuner picks for the N = 262,144 rotor wake on MI355X
-# (used with --no-tune, e.g. under a profiler): binary64 one-sided 256 x 2
-# targets, unroll 2, 32 source chunks (105.4 against 107.2 ms per RHS at the
-# policy's 16); binary64 symmetric 64 x 2 (blocks of 128 targets, whose
-# smaller boxes make more source tiles far), 512 chunks and strips (80.0
-# against 82.2 at the policy's, 86.3 for 256 x 1); compensated binary32
-# symmetric 256 x 1, 256 chunks (46.4 against 48.4 at 128, 49.0 for 512 x 1).
+# (used with --no-tune, e.g. under a profiler): binary64 one-sided 256 x 1
+# target, unroll 4, 24 source chunks (103.6 ms per RHS; 103.7 at 32 chunks,
+# 105.7 for 256 x 2, unroll 2, at the policy's 16); binary64 symmetric 256 x 2,
+# 256 chunks and strips (73.5 against 75.3 at the policy's 128); compensated
+# binary32 symmetric 256 x 1, 256 chunks (46.7 against 48.7 at 128, 49.8 for
+# 512 x 1).
 MEASURED_BEST = {
-    "fp64": {"block": 256, "targets_per_thread": 2, "unroll": 2, "scheme": 1, "chunks": 32},
-    "fp64_symmetric": {"block": 64, "targets_per_thread": 2, "unroll": 0, "scheme": 2,
-                       "chunks": 512},
+    "fp64": {"block": 256, "targets_per_thread": 1, "unroll": 4, "scheme": 1, "chunks": 24},
+    "fp64_symmetric": {"block": 256, "targets_per_thread": 2, "unroll": 0, "scheme": 2,
+                       "chunks": 256},
     "fp32c": {"block": 256, "targets_per_thread": 1, "unroll": 0, "scheme": 2, "chunks": 256},
 }
 
