@@ -8,7 +8,7 @@ root and a short polynomial, with Zs and the PSE terms dropped (below binary64
 rounding there).  The oracle evaluates the reference's full kernel for every
 pair (highorder.h:20-60, gaussian.h:20-58, interaction_utils.h:429-481).
 Checked here: a particle set in which most of every velocity comes from far
-pairs (both kernels, both precisions, both binary64 schemes), pairs of clusters whose boxes sit just
+pairs (both kernels, both precisions, both schemes of each), pairs of clusters whose boxes sit just
 below, at and above the binary64 threshold, and that the far choice leaves the
 one-sided variants bitwise equal.  Bars: binary64 1e-10 per particle (the
 parity bar; the far form is measured at rounding level, printed); binary32
@@ -59,13 +59,16 @@ def test_far_sparse_lattice_both_precisions(kernel):
     esv, esd = per_particle_rel(vs, rv), per_particle_rel(ds, rd)
     v32, d32 = _rates(w, nu, pa.FP32C, kernel)
     e32v, e32d = rel_rms(v32, rv), rel_rms(d32, rd)
+    vo, do = _rates(w, nu, pa.FP32C, kernel, scheme=pa.SCHEME_ONESIDED)
+    eov, eod = rel_rms(vo, rv), rel_rms(do, rd)
     print(f"  far pairs' share of the velocity {share:.2f}; binary64 per particle: velocity "
           f"{e64v:.2e} vorticity rate {e64d:.2e} (symmetric {esv:.2e} {esd:.2e}); binary32 "
-          f"rel RMS: {e32v:.2e} {e32d:.2e}", flush=True)
+          f"rel RMS: {e32v:.2e} {e32d:.2e} (one-sided {eov:.2e} {eod:.2e})", flush=True)
     assert share > 0.5
     assert e64v < TOL64 and e64d < TOL64, (e64v, e64d)
     assert esv < TOL64 and esd < TOL64, (esv, esd)
     assert e32v < TOL32 and e32d < TOL32, (e32v, e32d)
+    assert eov < TOL32 and eod < TOL32, (eov, eod)
 
 
 def _two_clusters(gap, core=0.05, seed=11):
