@@ -14,9 +14,12 @@
 //              Q  = [(0.25/pi) erf(sqrt(t/2)) t^-1/2 - Zs] / t
 //   both     : F' = (Zs - 3Q) / t
 // The PSE sum is regrouped, sum k (v_i a_j - v_j a_i) = v_i sum k a_j -
-// a_i sum k v_j, which saves four operations per pair and loses little (the
-// two sums cancel by a few digits at most).  The stretching cross product is
-// kept per pair: a_i x sum q a_j cancels when the vorticities are nearly
+// a_i sum k v_j, which saves four operations per pair and loses little in
+// binary64 (the two sums cancel by a few digits at most); the binary32 passes
+// add close pairs' terms as k (a_j - a_i), k (v_j - v_i), the differences
+// formed in binary64, which the same epilogue turns into the same sum.  Far
+// pairs take the kernel in a far-field form (below).  The stretching cross
+// product is kept per pair: a_i x sum q a_j cancels when the vorticities are nearly
 // parallel, as they are along a vortex filament (on a rotor wake in binary64
 // the stretching error grows from 1e-13 to 1.4e-11; in binary32 it would be
 // far above 1e-6).  The velocity stays per pair for the same reason
