@@ -600,12 +600,24 @@ def bench_rotor_growth(pa, gen, args):
            "radius": W._radius, "volume": W._volume}
     S.close()
     err = split_error_for(pa, fin, 1e-4, uinf=(0.0, 0.0, -0.05))
+    # both paths' rates at the grown state against the oracle's on 512
+    # sampled targets (each against all sources)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc  # the checker
+    ww = dict(fin, n=n, nu=1e-4, uinf=(0.0, 0.0, -0.05))
+    gpu = {}
+    for name, prec in (("fp64", pa.FP64), ("fp32c", pa.FP32C)):
+        Sx = pa.Interaction(pa.Wake.from_dict(ww), nu=1e-4, uinf=ww["uinf"], precision=prec)
+        gpu[name] = gpu_terms(Sx, ww)
+        Sx.close()
+    chk = oracle_check(orc, ww, gpu, sample_targets(n, 512))
     return {"metric": "rotor wake grown to N particles, FP32c pairs, FP64 state: pair-interactions/s",
             "value": pairs / sec, "unit": "pair-interactions/s", "n_gpus": 1, "steps": shed,
             "dtype": "f32c", "config": {"workload": f"rotor_growth_{n}", "final_n": n,
                                         "dt": LONG_DT}, "seconds": sec,
             "total_vorticity": d["total_vorticity"].tolist(),
-            "fp32c_rel_rms_err_vs_fp64_end": err}
+            "fp32c_rel_rms_err_vs_fp64_end": err,
+            "oracle_check_end": chk}
 
 
 def bench_batch(pa, gen, args):
