@@ -145,6 +145,45 @@ def test_far_choice_far_from_origin():
     print("  " + "; ".join(out), flush=True)
 
 
+@pytest.mark.parametrize("cut", [27, 63])
+def test_far_ragged_end(cut):
+    """The sparse lattice with its last sub-cube cut short (N = 4,096 - cut):
+    the job's last 64-particle group is partial (always the full form; its
+    targets past N repeat particle N - 1 in the box), every other pair of
+    sub-cubes is far.  Both precisions, both schemes, against the oracle, and
+    the one-sided binary64 variants bitwise equal."""
+    w = sparse_lattice()
+    n = w["n"] - cut
+    w = {k: (v[:n] if isinstance(v, np.ndarray) else v) for k, v in w.items()}
+    w["n"] = n
+    nu = 1e-3
+    print(f"case far ragged end N={n}", flush=True)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL)
+    out = []
+    for prec, sch in ((pa.FP64, pa.SCHEME_ONESIDED), (pa.FP64, pa.SCHEME_SYMMETRIC),
+                      (pa.FP32C, pa.SCHEME_ONESIDED), (pa.FP32C, pa.SCHEME_SYMMETRIC)):
+        v, d = _rates(w, nu, prec, scheme=sch)
+        if prec == pa.FP64:
+            e = (per_particle_rel(v, rv), per_particle_rel(d, rd))
+            assert e[0] < TOL64 and e[1] < TOL64, (prec, sch, e)
+        else:
+            e = (rel_rms(v, rv), rel_rms(d, rd))
+            assert e[0] < TOL32 and e[1] < TOL32, (prec, sch, e)
+        out.append(f"{prec}/{sch}: {e[0]:.2e} {e[1]:.2e}")
+    print("  " + "; ".join(out), flush=True)
+    ref = None
+    for block, tpt in ((64, 1), (128, 2), (256, 1), (256, 2)):
+        S = pa.Interaction(pa.Wake.from_dict(w), nu=nu, precision=pa.FP64, block=block,
+                           chunk=1024, targets_per_thread=tpt, unroll=2,
+                           scheme=pa.SCHEME_ONESIDED)
+        S.solve()
+        r = S.getRates()
+        S.close()
+        if ref is None:
+            ref = r
+        assert np.array_equal(r, ref), (block, tpt)
+
+
 def test_far_onesided_variants_bitwise_identical():
     """The far choice is made per pair of 64-particle groups of the job, so
     every one-sided binary64 variant (block, targets per thread, unroll) gives
