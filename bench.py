@@ -15,6 +15,8 @@ Prints one JSON line:
               stream, against the FP64 vector peak
   drift       total vorticity, linear impulse and kinetic energy before the
               warm-up steps and after the timed steps, per path
+  boundary    the binary64 rate across the host boundary: state uploaded, one
+              RHS, rates downloaded (as pawan's setStates / solve / getRates)
   cpu_baseline  the OpenMP CPU oracle (pawan's __parallel loop restated, with
               its per-pair heap temporaries) on a stratified sample of source
               rows of the same workload, on this host's cores; and the
