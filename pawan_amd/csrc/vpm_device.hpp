@@ -19,11 +19,14 @@
 // add close pairs' terms as k (a_j - a_i), k (v_j - v_i), the differences
 // formed in binary64, which the same epilogue turns into the same sum.  Far
 // pairs take the kernel in a far-field form (below).  The stretching cross
-// product is kept per pair: a_i x sum q a_j cancels when the vorticities are nearly
-// parallel, as they are along a vortex filament (on a rotor wake in binary64
-// the stretching error grows from 1e-13 to 1.4e-11; in binary32 it would be
-// far above 1e-6).  The velocity stays per pair for the same reason
-// (sum q a_j x x_i - sum q a_j x x_j cancels by |x| / |d|).
+// product of close pairs is kept per pair: a_i x sum q a_j over all sources
+// cancels when the vorticities are nearly parallel, as they are along a
+// vortex filament (on a rotor wake in binary64 the stretching error grows
+// from 1e-13 to 1.4e-11; in binary32 it would be far above 1e-6).  Their
+// velocity stays per pair for the same reason (sum q a_j x x_i -
+// sum q a_j x x_j cancels by |x| / |d|).  The binary64 one-sided pass
+// regroups only far pairs, per 64-source group and about the group's first
+// particle, where both ratios are bounded (far groups, by moments, below).
 // Each work item (target tile x source chunk) writes its NCOMP partial sums to
 // its own slot; a second kernel adds the chunks in a fixed order and applies
 // the epilogue, so results never depend on scheduling.
@@ -236,30 +239,52 @@ __device__ __forceinline__ void kernel64_far(double r2, double &Q, double &itm) 
     }
 }
 
-// One far pair (t >= T_FAR64) of a uniform-core job in binary64: the far-
-// field kernel, no PSE terms, and the stretching written as
-// q (e + (F'/Q)(d.e) d), one operation fewer.  43 arithmetic operations and
-// one v_rsq_f64 against 54 and two for the full pair.
+// ------------------------------------------------- far groups, by moments
+// The far pairs of a target with one 64-source group G (first particle o,
+// x' = x - o) in binary64, regrouped exactly:
+//   velocity    sum q a_j x d       = S x x'_i - sum q m_j
+//   stretching  sum q a_i x a_j     = a_i x S
+//               sum F'(d.e) d,  d.e = d.(a_i x a_j) = a_j.p_i - a_i.m_j
+// with S = sum_G q a_j, m_j = a_j x x'_j (per source, formed once while the
+// tile is staged) and p_i = x'_i x a_i (per target and group).  Each term is
+// at most |x'|/|d| larger than the sum it stands for; a group takes this
+// form only when its extent is at most sqrt(T_EXT64) sigma and the pair's
+// boxes are at least sqrt(T_FAR64) apart, so that the rounding of the far
+// sums grows by a bounded factor (at most 9; 1.5 measured on the rotor wake),
+// not with the distance from the origin.
+constexpr float T_EXT64 = 16384.0f;  // (128 sigma)^2
+
+// Moment m_j of a staged source about its group's first particle (32 bytes,
+// read as 2 x 16 B).
+struct __align__(16) Mom64 {
+    double x, y, z, pad;
+};
+
+// One far pair of a uniform-core job (t >= T_FAR64) in the moment form: the
+// far-field kernel and no PSE terms.  pv = p_i; S accumulates q a_j.  35
+// arithmetic operations and one v_rsq_f64 against 54 and two for the full
+// pair.
 template <int KIND>
-__device__ __forceinline__ void pair64_far(double xi, double yi, double zi, double aix, double aiy,
-                                           double aiz, const Rec64 &s, Acc64 &a) {
+__device__ __forceinline__ void pair64_mom(double xi, double yi, double zi, double aix, double aiy,
+                                           double aiz, const double *pv, const Rec64 &s,
+                                           const Mom64 &m, Acc64 &a, double *S) {
     const double dx = xi - s.x, dy = yi - s.y, dz = zi - s.z;
     const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
     double Q, itm;
     kernel64_far<KIND>(r2, Q, itm);
-    const double cx = __builtin_fma(s.ay, dz, -s.az * dy);
-    const double cy = __builtin_fma(s.az, dx, -s.ax * dz);
-    const double cz = __builtin_fma(s.ax, dy, -s.ay * dx);
-    a.v[0] = __builtin_fma(Q, cx, a.v[0]);
-    a.v[1] = __builtin_fma(Q, cy, a.v[1]);
-    a.v[2] = __builtin_fma(Q, cz, a.v[2]);
-    const double ex = __builtin_fma(aiy, s.az, -aiz * s.ay);
-    const double ey = __builtin_fma(aiz, s.ax, -aix * s.az);
-    const double ez = __builtin_fma(aix, s.ay, -aiy * s.ax);
-    const double k = itm * __builtin_fma(dz, ez, __builtin_fma(dy, ey, dx * ex));
-    a.v[3] = __builtin_fma(Q, __builtin_fma(k, dx, ex), a.v[3]);
-    a.v[4] = __builtin_fma(Q, __builtin_fma(k, dy, ey), a.v[4]);
-    a.v[5] = __builtin_fma(Q, __builtin_fma(k, dz, ez), a.v[5]);
+    // d.(a_i x a_j) = a_j.p_i - a_i.m_j
+    double tr = __builtin_fma(s.az, pv[2], __builtin_fma(s.ay, pv[1], s.ax * pv[0]));
+    tr = __builtin_fma(-aiz, m.z, __builtin_fma(-aiy, m.y, __builtin_fma(-aix, m.x, tr)));
+    const double qk = Q * (itm * tr);  // F' (d.e)
+    a.v[0] = __builtin_fma(-Q, m.x, a.v[0]);
+    a.v[1] = __builtin_fma(-Q, m.y, a.v[1]);
+    a.v[2] = __builtin_fma(-Q, m.z, a.v[2]);
+    a.v[3] = __builtin_fma(qk, dx, a.v[3]);
+    a.v[4] = __builtin_fma(qk, dy, a.v[4]);
+    a.v[5] = __builtin_fma(qk, dz, a.v[5]);
+    S[0] = __builtin_fma(Q, s.ax, S[0]);
+    S[1] = __builtin_fma(Q, s.ay, S[1]);
+    S[2] = __builtin_fma(Q, s.az, S[2]);
 }
 
 // Smallest and largest of v over the wave's 64 lanes, in every lane.
@@ -275,6 +300,13 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 __device__ __forceinline__ float uniform_f(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+// v of the wave's first active lane, in every lane.
+__device__ __forceinline__ double first_lane(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
 }
 
 // Bounding box of the 64 points (x, y, z) held one per lane, in binary32, in
@@ -406,20 +438,36 @@ __device__ __forceinline__ int next_item(int *counter, int *slot, int &k, int ni
 
 // Sources [j0, j0 + jn) of a staged tile (first source s0) against the
 // thread's TPT targets of a uniform-core job; target slot p takes them in the
-// far form when bit p of FM is set.  DIAG: the tile holds some of the block's
+// far form by moments when bit p of FM is set (a full group: mom holds its
+// moments, org its first particle).  DIAG: the tile holds some of the block's
 // own targets, and a pair j == i is skipped (a far group is never the
 // target's own).
 template <int KIND, int TPT, int UNR, unsigned FM, bool DIAG>
-__device__ __forceinline__ void group64(const Rec64 *tile, int j0, int jn, const Rec64 *me,
-                                        Acc64 *acc, double inv_s2, int s0, const int *idx) {
+__device__ __forceinline__ void group64(const Rec64 *tile, const Mom64 *mom, const double *org,
+                                        int j0, int jn, const Rec64 *me, Acc64 *acc,
+                                        double inv_s2, int s0, const int *idx) {
+    double pv[TPT][3], S[TPT][3];
+    if constexpr (FM != 0) {
+#pragma unroll
+        for (int p = 0; p < TPT; ++p)
+            if ((FM >> p) & 1u) {
+                const double ux = me[p].x - org[0], uy = me[p].y - org[1], uz = me[p].z - org[2];
+                pv[p][0] = __builtin_fma(uy, me[p].az, -uz * me[p].ay);  // x'_i x a_i
+                pv[p][1] = __builtin_fma(uz, me[p].ax, -ux * me[p].az);
+                pv[p][2] = __builtin_fma(ux, me[p].ay, -uy * me[p].ax);
+                S[p][0] = S[p][1] = S[p][2] = 0.0;
+            }
+    }
 #pragma unroll UNR
     for (int j = j0; j < j0 + jn; ++j) {
         const Rec64 sj = tile[j];
+        Mom64 mj;
+        if constexpr (FM != 0) mj = mom[j];
 #pragma unroll
         for (int p = 0; p < TPT; ++p) {
             if ((FM >> p) & 1u) {
-                pair64_far<KIND>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay, me[p].az, sj,
-                                 acc[p]);
+                pair64_mom<KIND>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay, me[p].az, pv[p],
+                                 sj, mj, acc[p], S[p]);
             } else {
                 if (DIAG && s0 + j == idx[p]) continue;
                 pair64<KIND, true>(me[p].x, me[p].y, me[p].z, me[p].ax, me[p].ay, me[p].az,
@@ -427,23 +475,38 @@ __device__ __forceinline__ void group64(const Rec64 *tile, int j0, int jn, const
             }
         }
     }
+    if constexpr (FM != 0) {
+        // velocity + S x x'_i, stretching + a_i x S
+#pragma unroll
+        for (int p = 0; p < TPT; ++p)
+            if ((FM >> p) & 1u) {
+                const double ux = me[p].x - org[0], uy = me[p].y - org[1], uz = me[p].z - org[2];
+                Acc64 &a = acc[p];
+                a.v[0] = __builtin_fma(S[p][1], uz, __builtin_fma(-S[p][2], uy, a.v[0]));
+                a.v[1] = __builtin_fma(S[p][2], ux, __builtin_fma(-S[p][0], uz, a.v[1]));
+                a.v[2] = __builtin_fma(S[p][0], uy, __builtin_fma(-S[p][1], ux, a.v[2]));
+                a.v[3] = __builtin_fma(me[p].ay, S[p][2], __builtin_fma(-me[p].az, S[p][1], a.v[3]));
+                a.v[4] = __builtin_fma(me[p].az, S[p][0], __builtin_fma(-me[p].ax, S[p][2], a.v[4]));
+                a.v[5] = __builtin_fma(me[p].ax, S[p][1], __builtin_fma(-me[p].ay, S[p][0], a.v[5]));
+            }
+    }
 }
 
 // group64 for the far mask fm (one bit per target slot, wave-uniform).
 template <int KIND, int TPT, int UNR, bool DIAG>
-__device__ __forceinline__ void group64_fm(unsigned fm, const Rec64 *tile, int j0, int jn,
-                                           const Rec64 *me, Acc64 *acc, double inv_s2, int s0,
-                                           const int *idx) {
+__device__ __forceinline__ void group64_fm(unsigned fm, const Rec64 *tile, const Mom64 *mom,
+                                           const double *org, int j0, int jn, const Rec64 *me,
+                                           Acc64 *acc, double inv_s2, int s0, const int *idx) {
     constexpr unsigned ALL = (1u << TPT) - 1;
     if (fm == 0)
-        group64<KIND, TPT, UNR, 0, DIAG>(tile, j0, jn, me, acc, inv_s2, s0, idx);
+        group64<KIND, TPT, UNR, 0, DIAG>(tile, mom, org, j0, jn, me, acc, inv_s2, s0, idx);
     else if (fm == ALL)
-        group64<KIND, TPT, UNR, ALL, DIAG>(tile, j0, jn, me, acc, inv_s2, s0, idx);
+        group64<KIND, TPT, UNR, ALL, DIAG>(tile, mom, org, j0, jn, me, acc, inv_s2, s0, idx);
     else if constexpr (TPT == 2) {
         if (fm == 1)
-            group64<KIND, TPT, UNR, 1, DIAG>(tile, j0, jn, me, acc, inv_s2, s0, idx);
+            group64<KIND, TPT, UNR, 1, DIAG>(tile, mom, org, j0, jn, me, acc, inv_s2, s0, idx);
         else
-            group64<KIND, TPT, UNR, 2, DIAG>(tile, j0, jn, me, acc, inv_s2, s0, idx);
+            group64<KIND, TPT, UNR, 2, DIAG>(tile, mom, org, j0, jn, me, acc, inv_s2, s0, idx);
     }
 }
 
@@ -456,16 +519,19 @@ __device__ __forceinline__ void group64_fm(unsigned fm, const Rec64 *tile, int j
 // TPT targets per thread.  UNR is the unroll of the source loop.  Neither TPT
 // nor UNR changes the order of any target's sums.  In a uniform-core job each
 // (64-target group, 64-source group) whose bounding boxes are at least
-// sqrt(T_FAR64) apart runs in the far form: the groups are aligned to 64
-// particles of the job, so the choice is the same for every variant.
+// sqrt(T_FAR64) apart, the source group's no wider than sqrt(T_EXT64), runs in
+// the far form by moments: the groups are aligned to 64 particles of the job,
+// so the choice is the same for every variant.
 template <int KIND, int TB, int TPT, int UNR>
-__global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ jobs,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TPT == 1 ? 3 : 2))) void pair64_kernel(const JobDev *__restrict__ jobs,
                                                     const Item *__restrict__ items,
                                                     const int *__restrict__ nitems_p,
                                                     int *__restrict__ counter) {
     constexpr int ST = TB;
     constexpr int TW = TB * TPT;  // targets per tile
     __shared__ Rec64 tile[ST];
+    __shared__ Mom64 mom[ST];           // moments of full groups' sources
+    __shared__ double sorg[ST / 64][3]; // first particle of each 64-source group
     __shared__ float sbox[ST / 64][6];  // boxes of the staged tile's 64-source groups
     __shared__ int slot;
     const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
@@ -503,18 +569,31 @@ __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ j
                 double2 *dst = (double2 *)tile;
                 for (int k = threadIdx.x; k < cnt * 4; k += TB) dst[k] = src[k];
             }
-            // wave w: the box of source group w, when full (read beside the
-            // copy, shared through LDS)
+            // wave w: the box of source group w and its sources' moments
+            // about its first particle, when full (read beside the copy,
+            // shared through LDS)
             if (uniform && (w + 1) * 64 <= cnt) {
-                const Rec64 &q = rec[s0 + w * 64 + lane];
+                const Rec64 q = rec[s0 + w * 64 + lane];
                 float lo[3], hi[3];
                 wave_box(q.x, q.y, q.z, lo, hi);
-                if (lane == 0)
+                const double ox = first_lane(q.x), oy = first_lane(q.y), oz = first_lane(q.z);
+                const double ux = q.x - ox, uy = q.y - oy, uz = q.z - oz;
+                Mom64 m;  // a_j x x'_j
+                m.x = __builtin_fma(q.ay, uz, -q.az * uy);
+                m.y = __builtin_fma(q.az, ux, -q.ax * uz);
+                m.z = __builtin_fma(q.ax, uy, -q.ay * ux);
+                m.pad = 0.0;
+                mom[w * 64 + lane] = m;
+                if (lane == 0) {
+                    sorg[w][0] = ox;
+                    sorg[w][1] = oy;
+                    sorg[w][2] = oz;
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         sbox[w][c] = lo[c];
                         sbox[w][3 + c] = hi[c];
                     }
+                }
             }
             __syncthreads();
             // a tile holding some of the block's own targets skips j == i
@@ -522,26 +601,31 @@ __global__ __launch_bounds__(TB) void pair64_kernel(const JobDev *__restrict__ j
             if (uniform) {
                 // 64 sources at a time; target slot p takes them in the far
                 // form when its group's box and theirs are far apart (never
-                // its own group: the gap is 0)
+                // its own group: the gap is 0) and theirs is narrow
                 for (int g0 = 0; g0 < cnt; g0 += 64) {
                     const int gn = min(64, cnt - g0);
                     unsigned fm = 0;
                     if (gn == 64) {
-                        float slo[3], shi[3];
+                        float slo[3], shi[3], ext = 0.0f;
 #pragma unroll
                         for (int c = 0; c < 3; ++c) {
                             slo[c] = uniform_f(sbox[g0 >> 6][c]);
                             shi[c] = uniform_f(sbox[g0 >> 6][3 + c]);
+                            ext = __builtin_fmaf(shi[c] - slo[c], shi[c] - slo[c], ext);
                         }
+                        if (ext <= T_EXT64)
 #pragma unroll
-                        for (int p = 0; p < TPT; ++p)
-                            if (box_gap2(tlo[p], thi[p], slo, shi) >= T_FAR64) fm |= 1u << p;
+                            for (int p = 0; p < TPT; ++p)
+                                if (box_gap2(tlo[p], thi[p], slo, shi) >= T_FAR64) fm |= 1u << p;
                         fm = __builtin_amdgcn_readfirstlane(fm);
                     }
+                    const double *org = sorg[g0 >> 6];
                     if (diag)
-                        group64_fm<KIND, TPT, UNR, true>(fm, tile, g0, gn, me, acc, inv_s2, s0, idx);
+                        group64_fm<KIND, TPT, UNR, true>(fm, tile, mom, org, g0, gn, me, acc,
+                                                         inv_s2, s0, idx);
                     else
-                        group64_fm<KIND, TPT, UNR, false>(fm, tile, g0, gn, me, acc, inv_s2, s0, idx);
+                        group64_fm<KIND, TPT, UNR, false>(fm, tile, mom, org, g0, gn, me, acc,
+                                                          inv_s2, s0, idx);
                 }
             } else if (!diag) {
 #pragma unroll UNR
