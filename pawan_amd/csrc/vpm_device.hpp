@@ -150,8 +150,9 @@ __device__ __forceinline__ void kernel32(float t, float rt, float &Q, float &Fp,
 // F'); GAUSSIAN Q = (0.25/pi) t^-1.5 (erf = 1 and exp(-t/2) = 0 in binary32).
 // F' = -3 Q / t for both.
 constexpr float T_FAR32 = 128.0f;
+// kernel32_farq gives Q and itm = F'/Q = -3/t instead of F'.
 template <int KIND>
-__device__ __forceinline__ void kernel32_far(float rt, float &Q, float &Fp) {
+__device__ __forceinline__ void kernel32_farq(float rt, float &Q, float &itm) {
     const float u = rt * rt;  // 1/t
     if (KIND == KIND_HIGHORDER) {
         const float u2 = u * u;
@@ -162,7 +163,13 @@ __device__ __forceinline__ void kernel32_far(float rt, float &Q, float &Fp) {
     } else {
         Q = ((float)K_CPHI * rt) * u;
     }
-    Fp = Q * (-3.0f * u);
+    itm = -3.0f * u;
+}
+template <int KIND>
+__device__ __forceinline__ void kernel32_far(float rt, float &Q, float &Fp) {
+    float itm;
+    kernel32_farq<KIND>(rt, Q, itm);
+    Fp = Q * itm;
 }
 
 struct Acc64 {
@@ -974,9 +981,9 @@ __device__ __forceinline__ void pair32s(const PT &me, const PT &s, bool valid, A
     const float3 d = disp32(me, s);
     const float dx = d.x, dy = d.y, dz = d.z;
     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-    float Q, Fp, Zs = 0.0f;
+    float Q, Fp, Zs = 0.0f;  // far-field form: Fp holds F'/Q
     if (!DIFF) {
-        kernel32_far<KIND>(__builtin_amdgcn_rsqf(r2), Q, Fp);
+        kernel32_farq<KIND>(__builtin_amdgcn_rsqf(r2), Q, Fp);
     } else if (UNIFORM) {
         const float t = MASKED ? (valid ? r2 : 1.0f) : r2;
         kernel32<KIND>(t, __builtin_amdgcn_rsqf(t), Q, Fp, Zs);
@@ -1007,10 +1014,19 @@ __device__ __forceinline__ void pair32s(const PT &me, const PT &s, bool valid, A
     // the same with e and d negated, i.e. minus it
     const float3 e = cross32(me, s);
     const float ex = e.x, ey = e.y, ez = e.z;
-    const float fd = Fp * __builtin_fmaf(dz, ez, __builtin_fmaf(dy, ey, dx * ex));
-    const float sx = __builtin_fmaf(fd, dx, Q * ex);
-    const float sy = __builtin_fmaf(fd, dy, Q * ey);
-    const float sz = __builtin_fmaf(fd, dz, Q * ez);
+    const float de = __builtin_fmaf(dz, ez, __builtin_fmaf(dy, ey, dx * ex));
+    float sx, sy, sz;
+    if (!DIFF) {  // q (e + (F'/q)(d.e) d): one operation fewer
+        const float k = Fp * de;
+        sx = Q * __builtin_fmaf(k, dx, ex);
+        sy = Q * __builtin_fmaf(k, dy, ey);
+        sz = Q * __builtin_fmaf(k, dz, ez);
+    } else {
+        const float fd = Fp * de;
+        sx = __builtin_fmaf(fd, dx, Q * ex);
+        sy = __builtin_fmaf(fd, dy, Q * ey);
+        sz = __builtin_fmaf(fd, dz, Q * ez);
+    }
     a.v[3] += sx;
     a.v[4] += sy;
     a.v[5] += sz;
@@ -1097,8 +1113,10 @@ __device__ __forceinline__ void sym_steps(const SrcTile &tile, int s_begin, int 
                 pair32s<KIND, UNIFORM, true, false>(me[p], sj, valid, a[p], b);
             }
         }
+        // the caller starts jacc at zero: the first step takes b as it is
 #pragma unroll
-        for (int m = 0; m < (DIFF ? NCOMP_S : 6); ++m) jacc.v[m] = rot_add(jacc.v[m], b.v[m]);
+        for (int m = 0; m < (DIFF ? NCOMP_S : 6); ++m)
+            jacc.v[m] = u == 0 ? b.v[m] : rot_add(jacc.v[m], b.v[m]);
     }
 }
 
