@@ -203,3 +203,39 @@ def test_far_onesided_variants_bitwise_identical():
                 if ref is None:
                     ref = r
                 assert np.array_equal(r, ref), (block, tpt, unroll)
+
+
+@pytest.mark.parametrize("offset", [0.0, 2.0 ** 20])
+@pytest.mark.parametrize("length", [120.0, 136.0])
+def test_far_wide_source_group(length, offset):
+    """A compact 64-particle cluster and a 64-particle line `length` core
+    radii long, 40 core radii beyond the cluster's box: the binary64 one-sided
+    pass adds a far group by moments only when it is at most 128 core radii
+    wide (the line at 120: moments, at 136: the full form; the cluster as a
+    source group: moments either way), with every regrouped term at most
+    1 + 2 x 128 / 32 times the per-pair term it stands for.  The whole set is
+    also placed 2^20 core radii from the origin, where the moments about each
+    group's first particle keep the same bound (sigma = 1/16, so that the
+    records' coordinates in units of sigma are exact there, as the oracle's
+    are).  Both against the oracle."""
+    core = 1.0 / 16.0
+    u = gen.uniform(23, 12 * 64).reshape(128, 6)
+    a = 0.5 * core * u[:64, :3]
+    b = np.zeros((64, 3))
+    b[:, 1] = np.linspace(0.0, length * core, 64)
+    b[:, 2] = 0.25 * core * u[64:, 2]
+    b[:, 0] = a[:, 0].max() + 40.0 * core + 0.25 * core * u[64:, 0]
+    pos = np.concatenate([a, b]) + offset * core
+    vort = 0.01 * (u[:, 3:6] - 0.5)
+    n = 128
+    w = {"pos": np.ascontiguousarray(pos), "vort": np.ascontiguousarray(vort),
+         "radius": np.full(n, core), "volume": np.full(n, 4.0 / 3.0 * np.pi * core ** 3), "n": n}
+    nu = 1e-3
+    print(f"case far wide source group length={length} offset={offset}", flush=True)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL)
+    v64, d64 = _rates(w, nu, pa.FP64)
+    e64v, e64d = per_particle_rel(v64, rv), per_particle_rel(d64, rd)
+    # the line's velocities come from its own pairs and from the cluster
+    # (a far source group); the cluster's from itself and from the line
+    print(f"  binary64 per particle: velocity {e64v:.2e} vorticity rate {e64d:.2e}", flush=True)
+    assert e64v < TOL64 and e64d < TOL64, (e64v, e64d)
