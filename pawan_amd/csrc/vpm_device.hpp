@@ -1137,13 +1137,16 @@ __device__ __forceinline__ bool far_boxes(const float *blo, const float *bhi, co
 }
 
 // Occupancy: the HIGHORDER kernel with one target per thread is held to 128
-// VGPRs, i.e. 4 waves per SIMD (the LDS of a 256-thread block, 29 KB, allows
-// 5 blocks per CU).  Left to itself the compiler takes 148 (3 waves).  The
+// VGPRs, i.e. 4 waves per SIMD (the LDS of a 256-thread block, 37 KB, allows
+// 4 blocks per CU).  Left to itself the compiler takes 148 (3 waves).  The
 // unrolled steps are unchanged (the limit costs no spill inside them); at
-// N = 262,144 the pass takes 65.3 ms instead of 68.0.  Five waves (96 VGPRs)
-// spill inside the steps and take 90 ms; two targets per thread at three
-// waves (168 VGPRs) take 74.4 ms instead of 72.0; GAUSSIAN spills at 128.
-// Other instantiations keep the compiler's choice.
+// N = 262,144 the pass took 65.3 ms instead of 68.0 (before far tiles).  Five
+// waves (96 VGPRs) spill inside the steps and take 90 ms; GAUSSIAN spills at
+// 128.  Two targets per thread issue fewer instructions per pair (51.75 per
+// far-field pair against 54.1: half the rotations and LDS reads) but need
+// 252 VGPRs: held to two waves per SIMD they take 49.7 ms against 45.3 (at
+// three, 74.4 against 72.0 before far tiles).  Other instantiations keep the
+// compiler's choice.
 template <int KIND, int TB, int TPT>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
     (KIND == KIND_HIGHORDER && TPT == 1) ? 4 : 1))) void pair32s_kernel(const JobDev *__restrict__ jobs,
