@@ -897,7 +897,8 @@ __device__ __forceinline__ float rot_add(float x, float y) {
 struct SrcTile {
     double2 xy[128], za[128], aa[128];  // (x, y), (z, ax), (ay, az)
     float4 f[128];                      // (fx, fy, fz, v)
-    float4 p[128];                      // (x, y, z, h) rounded to binary32: far tiles
+    float4 p[128];                      // (x - c, y - c, z - c, h) in binary32, c the centre of the
+                                        // block's box: far tiles
     float h[128];
     double vd[128];                     // binary64 volumes: PSE differences of close pairs
 };
@@ -915,8 +916,10 @@ struct Part32f {
 
 // A source tile is far from a target block when the distance between their
 // bounding boxes is at least K_FAR times the largest coordinate magnitude of
-// the two boxes.  Its pairs then run wholly in binary32: the displacement
-// from binary32 positions has a relative error below 2 eps / K_FAR (1e-6),
+// the two boxes, measured from the centre of the block's box, relative to
+// which a far tile's sources and the block's targets are rounded to binary32.
+// Its pairs then run wholly in binary32: the displacement from those binary32
+// positions has a relative error below 2 eps / K_FAR (1e-6),
 // and the stretching cross product from binary32 strengths loses accuracy
 // only in proportion to the pair's small weight.  Emulated against the
 // oracle, the far pairs add 4e-8 relative RMS to the stretching rates of the
@@ -1121,16 +1124,18 @@ __device__ __forceinline__ void sym_steps(const SrcTile &tile, int s_begin, int 
 }
 
 // Whether a tile whose box is [tlo, thi] is far from a block whose box is
-// [blo, bhi] (K_FAR); d2 receives the squared distance between the boxes.
+// [blo, bhi] (K_FAR, coordinates measured from ctr); d2 receives the squared
+// distance between the boxes.
 __device__ __forceinline__ bool far_boxes(const float *blo, const float *bhi, const float *tlo,
-                                          const float *thi, float &d2) {
+                                          const float *thi, const float *ctr, float &d2) {
     float m = 0.0f;
     d2 = 0.0f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float g = fmaxf(0.0f, fmaxf(tlo[c] - bhi[c], blo[c] - thi[c]));
         d2 = __builtin_fmaf(g, g, d2);
-        m = fmaxf(m, fmaxf(fmaxf(fabsf(blo[c]), fabsf(bhi[c])), fmaxf(fabsf(tlo[c]), fabsf(thi[c]))));
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(blo[c] - ctr[c]), fabsf(bhi[c] - ctr[c])),
+                           fmaxf(fabsf(tlo[c] - ctr[c]), fabsf(thi[c] - ctr[c]))));
     }
     const float km = K_FAR * m;
     return d2 >= km * km;
@@ -1227,6 +1232,24 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
             const int cnt = min(64, c1 - s0);
             const int st = s0 >> 6;  // source tile
             __syncthreads();
+            // the block's box (the waves' boxes were written before the
+            // barrier above) and its centre, from which a far tile's binary32
+            // positions are measured
+            float blo[3], bhi[3], ctr[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                blo[c] = bbw[0][c];
+                bhi[c] = bbw[0][3 + c];
+            }
+#pragma unroll
+            for (int ww = 1; ww < NW; ++ww)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    blo[c] = fminf(blo[c], bbw[ww][c]);
+                    bhi[c] = fmaxf(bhi[c], bbw[ww][3 + c]);
+                }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ctr[c] = 0.5f * (blo[c] + bhi[c]);
             if (threadIdx.x < 64) {  // stage twice; missing sources read as zeros
                 const int k = (int)threadIdx.x;
                 Rec32 r;
@@ -1236,7 +1259,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                     r.x = r.y = r.z = r.h = r.ax = r.ay = r.az = r.vd = 0.0;
                     r.fx = r.fy = r.fz = r.v = 0.0f;
                 }
-                const float px = (float)r.x, py = (float)r.y, pz = (float)r.z;
+                const float px = (float)(r.x - (double)ctr[0]), py = (float)(r.y - (double)ctr[1]),
+                            pz = (float)(r.z - (double)ctr[2]);
 #pragma unroll
                 for (int c = 0; c < 128; c += 64) {
                     tile.xy[k + c] = make_double2(r.x, r.y);
@@ -1248,28 +1272,25 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                     tile.vd[k + c] = r.vd;
                 }
                 // far test (full tiles only): the tile's box against the
-                // block's (the waves' boxes were written before the barrier
-                // above)
+                // block's
                 int f = 0;
                 if (cnt == 64) {
-                    float tlo[3], thi[3], blo[3], bhi[3];
+                    float tlo[3], thi[3];
                     wave_box(r.x, r.y, r.z, tlo, thi);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        blo[c] = bbw[0][c];
-                        bhi[c] = bbw[0][3 + c];
-                    }
-#pragma unroll
-                    for (int ww = 1; ww < NW; ++ww)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            blo[c] = fminf(blo[c], bbw[ww][c]);
-                            bhi[c] = fmaxf(bhi[c], bbw[ww][3 + c]);
-                        }
+                    // a tile whose pairs all have t >= T_FAR32 is far when its
+                    // coordinates measured from the block's centre are; a
+                    // closer one only when they are measured from the origin
+                    // (its stretching cross products, of pairs down to ~6
+                    // sigma on the rotor wake, would lose 40% of the binary32
+                    // stretching accuracy in binary32)
+                    const float zero[3] = {0.0f, 0.0f, 0.0f};
                     float d2;
-                    f = far_boxes(blo, bhi, tlo, thi, d2) ? 1 : 0;
+                    const bool far_c = far_boxes(blo, bhi, tlo, thi, ctr, d2);
                     // uniform core: coordinates are in units of sigma, t = rho^2
-                    if (uniform && d2 >= T_FAR32) f = f ? 2 : 3;
+                    if (uniform && d2 >= T_FAR32)
+                        f = far_c ? 2 : 3;
+                    else
+                        f = far_boxes(blo, bhi, tlo, thi, zero, d2) ? 1 : 0;
                 }
                 if (k == 0) tfar = f;
             }
@@ -1314,8 +1335,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                             Part32f mf[TPT];
 #pragma unroll
                             for (int p = 0; p < TPT; ++p)
-                                mf[p] = Part32f{(float)me[p].x, (float)me[p].y, (float)me[p].z,
-                                                me[p].fx, me[p].fy, me[p].fz, me[p].v, me[p].h};
+                                mf[p] = Part32f{(float)(me[p].x - (double)ctr[0]),
+                                                (float)(me[p].y - (double)ctr[1]),
+                                                (float)(me[p].z - (double)ctr[2]), me[p].fx, me[p].fy,
+                                                me[p].fz, me[p].v, me[p].h};
                             if (ffield)
                                 sym_steps<KIND, true, false, TPT, SBJ, false>(tile, sj, lane, mf, live,
                                                                               nt, diag, cnt, a, jacc);
