@@ -239,3 +239,28 @@ def test_far_wide_source_group(length, offset):
     # (a far source group); the cluster's from itself and from the line
     print(f"  binary64 per particle: velocity {e64v:.2e} vorticity rate {e64d:.2e}", flush=True)
     assert e64v < TOL64 and e64d < TOL64, (e64v, e64d)
+
+
+def test_far_lattice_away_from_origin():
+    """The sparse lattice moved 2^20 core radii from the origin (sigma = 1/4,
+    so that the records' coordinates in units of sigma are exact).  The
+    binary32 pass measures a far-field tile's coordinates from the centre of
+    the target block's box, so these tiles still run wholly in binary32; both
+    binary32 schemes and the binary64 pass against the oracle, as at the
+    origin."""
+    core = 0.25
+    w = sparse_lattice(core=core)
+    w["pos"] = w["pos"] + 2.0 ** 20 * core
+    nu = 1e-3
+    print(f"case far lattice at 2^20 sigma N={w['n']}", flush=True)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL)
+    v64, d64 = _rates(w, nu, pa.FP64)
+    e64v, e64d = per_particle_rel(v64, rv), per_particle_rel(d64, rd)
+    out = [f"binary64 {e64v:.2e} {e64d:.2e}"]
+    assert e64v < TOL64 and e64d < TOL64, (e64v, e64d)
+    for sch in (pa.SCHEME_SYMMETRIC, pa.SCHEME_ONESIDED):
+        v, d = _rates(w, nu, pa.FP32C, scheme=sch)
+        e = (rel_rms(v, rv), rel_rms(d, rd))
+        out.append(f"binary32 scheme {sch}: {e[0]:.2e} {e[1]:.2e}")
+        assert e[0] < TOL32 and e[1] < TOL32, (sch, e)
+    print("  " + "; ".join(out), flush=True)
