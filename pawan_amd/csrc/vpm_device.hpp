@@ -1277,20 +1277,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                 if (cnt == 64) {
                     float tlo[3], thi[3];
                     wave_box(r.x, r.y, r.z, tlo, thi);
-                    // a tile whose pairs all have t >= T_FAR32 is far when its
-                    // coordinates measured from the block's centre are; a
-                    // closer one only when they are measured from the origin
-                    // (its stretching cross products, of pairs down to ~6
-                    // sigma on the rotor wake, would lose 40% of the binary32
-                    // stretching accuracy in binary32)
-                    const float zero[3] = {0.0f, 0.0f, 0.0f};
                     float d2;
-                    const bool far_c = far_boxes(blo, bhi, tlo, thi, ctr, d2);
+                    f = far_boxes(blo, bhi, tlo, thi, ctr, d2) ? 1 : 0;
                     // uniform core: coordinates are in units of sigma, t = rho^2
-                    if (uniform && d2 >= T_FAR32)
-                        f = far_c ? 2 : 3;
-                    else
-                        f = far_boxes(blo, bhi, tlo, thi, zero, d2) ? 1 : 0;
+                    if (uniform && d2 >= T_FAR32) f = f ? 2 : 3;
                 }
                 if (k == 0) tfar = f;
             }
