@@ -148,17 +148,25 @@ __device__ __forceinline__ void kernel32(float t, float rt, float &Q, float &Fp,
 // HIGHORDER Q = K_HC32 g(u) t^-1/2 u^2 with g truncated after u^3 (the u^4
 // term is below 4e-9 relative) and Zs dropped (below 2e-12 relative in Q and
 // F'); GAUSSIAN Q = (0.25/pi) t^-1.5 (erf = 1 and exp(-t/2) = 0 in binary32).
-// F' = -3 Q / t for both.
+// F' = -3 Q / t for both.  DEG4: t >= T_FAR32B (rho >= 8 sigma), g truncated
+// after u^4 (the u^5 term is below 1.5e-9 relative, Zs below 1e-8 of Q);
+// Gaussian as before (exp(-32) is below 1e-13 of erf there).
 constexpr float T_FAR32 = 128.0f;
+constexpr float T_FAR32B = 64.0f;
 // kernel32_farq gives Q and itm = F'/Q = -3/t instead of F'.
-template <int KIND>
+template <int KIND, bool DEG4 = false>
 __device__ __forceinline__ void kernel32_farq(float rt, float &Q, float &itm) {
     const float u = rt * rt;  // 1/t
     if (KIND == KIND_HIGHORDER) {
         const float u2 = u * u;
-        // K_HC32 (1 - u/2 + u^3/2)
-        const float g = __builtin_fmaf(__builtin_fmaf((float)(0.5 * K_HC32), u2, (float)(-0.5 * K_HC32)), u,
-                                       (float)K_HC32);
+        float g;
+        if (DEG4)  // K_HC32 (1 - u/2 + u^3/2 - u^4)
+            g = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf((float)-K_HC32, u, (float)(0.5 * K_HC32)), u2,
+                                              (float)(-0.5 * K_HC32)),
+                               u, (float)K_HC32);
+        else  // K_HC32 (1 - u/2 + u^3/2)
+            g = __builtin_fmaf(__builtin_fmaf((float)(0.5 * K_HC32), u2, (float)(-0.5 * K_HC32)), u,
+                               (float)K_HC32);
         Q = (g * rt) * u2;
     } else {
         Q = ((float)K_CPHI * rt) * u;
@@ -977,7 +985,8 @@ __device__ __forceinline__ Part32 load_src(const SrcTile &t, int k, Part32) {
 // valid == false contributes exactly zero (its operands are finite).
 // DIFF == false: a far pair of a uniform-core job (t >= T_FAR32): the far-
 // field kernel and no PSE terms (sums 6-9 untouched on both sides).
-template <int KIND, bool UNIFORM, bool MASKED, bool FIRST, bool DIFF = true, class PT>
+template <int KIND, bool UNIFORM, bool MASKED, bool FIRST, bool DIFF = true, bool DEG4 = false,
+          class PT>
 __device__ __forceinline__ void pair32s(const PT &me, const PT &s, bool valid, Acc32s &a,
                                         Acc32s &b) {
     static_assert(DIFF || (UNIFORM && !MASKED), "the far form is for full tiles of uniform jobs");
@@ -986,7 +995,7 @@ __device__ __forceinline__ void pair32s(const PT &me, const PT &s, bool valid, A
     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
     float Q, Fp, Zs = 0.0f;  // far-field form: Fp holds F'/Q
     if (!DIFF) {
-        kernel32_farq<KIND>(__builtin_amdgcn_rsqf(r2), Q, Fp);
+        kernel32_farq<KIND, DEG4>(__builtin_amdgcn_rsqf(r2), Q, Fp);
     } else if (UNIFORM) {
         const float t = MASKED ? (valid ? r2 : 1.0f) : r2;
         kernel32<KIND>(t, __builtin_amdgcn_rsqf(t), Q, Fp, Zs);
@@ -1088,7 +1097,8 @@ __device__ __forceinline__ void pair32s(const PT &me, const PT &s, bool valid, A
 // PT: Part32, or Part32f for a far tile (never masked).  DIFF == false: a
 // far tile in the far-field form; sums 6-9 are neither added nor rotated (64
 // rotations of a sum that receives nothing leave it where it was).
-template <int KIND, bool UNIFORM, bool MASKED, int TPT, int SB, bool DIFF = true, class PT>
+template <int KIND, bool UNIFORM, bool MASKED, int TPT, int SB, bool DIFF = true, bool DEG4 = false,
+          class PT>
 __device__ __forceinline__ void sym_steps(const SrcTile &tile, int s_begin, int lane, const PT *me,
                                           const bool *live, const int *nt, const bool *diag,
                                           int cnt, Acc32s *a, Acc32s &jacc) {
@@ -1100,10 +1110,10 @@ __device__ __forceinline__ void sym_steps(const SrcTile &tile, int s_begin, int 
         const PT sj = load_src(tile, e0 + u, PT{});
         Acc32s b;
         if (!MASKED) {
-            pair32s<KIND, UNIFORM, false, true, DIFF>(me[0], sj, true, a[0], b);
+            pair32s<KIND, UNIFORM, false, true, DIFF, DEG4>(me[0], sj, true, a[0], b);
 #pragma unroll
             for (int p = 1; p < TPT; ++p)
-                pair32s<KIND, UNIFORM, false, false, DIFF>(me[p], sj, true, a[p], b);
+                pair32s<KIND, UNIFORM, false, false, DIFF, DEG4>(me[p], sj, true, a[p], b);
         } else {
             const bool src_ok = k < cnt;
 #pragma unroll
@@ -1168,7 +1178,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
     __shared__ float bbw[NW][6];            // per wave: box of its targets (lo xyz, hi xyz)
     __shared__ int tfar;                    // the staged tile is far from the block (1),
                                             // and t >= T_FAR32 for all its pairs (2); not
-                                            // far, but t >= T_FAR32 (3)
+                                            // far, but t >= T_FAR32 (3); far, with
+                                            // T_FAR32B <= t (4: one series term more)
     __shared__ int slot;
     const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int nitems = *nitems_p;
@@ -1280,7 +1291,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                     float d2;
                     f = far_boxes(blo, bhi, tlo, thi, ctr, d2) ? 1 : 0;
                     // uniform core: coordinates are in units of sigma, t = rho^2
-                    if (uniform && d2 >= T_FAR32) f = f ? 2 : 3;
+                    if (uniform && d2 >= T_FAR32)
+                        f = f ? 2 : 3;
+                    else if (uniform && f && d2 >= T_FAR32B)
+                        f = 4;
                 }
                 if (k == 0) tfar = f;
             }
@@ -1295,7 +1309,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
             // far: binary32 positions and strengths; ffield: the far-field
             // kernel and no PSE terms (binary64 displacement and cross product
             // unless also far)
-            const bool far = farv == 1 || farv == 2, ffield = farv >= 2;
+            const bool far = farv == 1 || farv == 2 || farv == 4, ffield = farv >= 2;
             bool live[TPT], diag[TPT], full = cnt == 64, any = false;
 #pragma unroll
             for (int p = 0; p < TPT; ++p) {
@@ -1329,7 +1343,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(
                                                 (float)(me[p].y - (double)ctr[1]),
                                                 (float)(me[p].z - (double)ctr[2]), me[p].fx, me[p].fy,
                                                 me[p].fz, me[p].v, me[p].h};
-                            if (ffield)
+                            if (farv == 4)
+                                sym_steps<KIND, true, false, TPT, SBJ, false, true>(
+                                    tile, sj, lane, mf, live, nt, diag, cnt, a, jacc);
+                            else if (ffield)
                                 sym_steps<KIND, true, false, TPT, SBJ, false>(tile, sj, lane, mf, live,
                                                                               nt, diag, cnt, a, jacc);
                             else if (uniform)

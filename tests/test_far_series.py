@@ -3,12 +3,13 @@ oracle's long-double evaluation of the reference's closed forms
 (highorder.h:20-60, gaussian.h:20-58; oracle orc_kernel_ld).
 
 Far pairs (t = rho^2/sigma^2 >= 1024 in the binary64 passes, >= 128 in the
-binary32 ones) take q and F from a truncated series in u = 1/t with Z dropped:
+binary32 ones, >= 64 in binary32 far tiles with one term more) take q and F
+from a truncated series in u = 1/t with Z dropped:
   HIGHORDER  (t + 1.5)(t + 1)^-2 = u g(u),  g(u) = sum_k (-1)^k (1 - k/2) u^k
              q = (0.25/pi) g(u) u^2 t^-1/2 / sigma^3,  F = -3 q / rho^2
   GAUSSIAN   q = (0.25/pi) t^-1.5 / sigma^3,  F = -3 q / rho^2
 This test evaluates those series in binary64 at the truncation the kernels use
-(u^4 for binary64, u^3 for binary32) and bounds their distance from the full
+(u^4 for binary64, u^3 for binary32, u^4 past t = 64) and bounds their distance from the full
 kernel where each threshold lets them in, and the size of the dropped Z; it
 also checks the series' coefficients against the expansion of the closed form
 by exact rational arithmetic, and that a series truncated one term earlier
@@ -22,7 +23,8 @@ import pytest
 
 import oracle as orc
 
-T64, T32 = 1024.0, 128.0
+T64, T32, T32B = 1024.0, 128.0, 64.0
+ZBOUND = {T64: 1e-14, T32: 1e-9, T32B: 1e-8}  # |Z| / |q| where Z is dropped
 
 
 def _kernel_ld(kind, t, sigma=0.05):
@@ -57,7 +59,7 @@ def test_series_coefficients_exact():
 
 
 @pytest.mark.parametrize("kind", [orc.HIGHORDER, orc.GAUSSIAN])
-@pytest.mark.parametrize("thresh,degree,tol", [(T64, 4, 4e-15), (T32, 3, 1e-8)])
+@pytest.mark.parametrize("thresh,degree,tol", [(T64, 4, 4e-15), (T32, 3, 1e-8), (T32B, 4, 2e-9)])
 def test_far_series_matches_full_kernel(kind, thresh, degree, tol):
     ts = thresh * np.exp(np.linspace(0.0, np.log(1e10), 400))
     eq = ef = zr = 0.0
@@ -71,10 +73,10 @@ def test_far_series_matches_full_kernel(kind, thresh, degree, tol):
         zr = max(zr, abs(Z) / abs(q))
     print(f"  kind {kind} t >= {thresh:g}: q {eq:.2e} F {ef:.2e}, Z/q {zr:.2e}")
     assert eq < tol and ef < tol
-    assert zr < (1e-14 if thresh == T64 else 1e-9)
+    assert zr < ZBOUND[thresh]
 
 
-@pytest.mark.parametrize("thresh,degree,tol", [(T64, 3, 4e-15), (T32, 2, 1e-8)])
+@pytest.mark.parametrize("thresh,degree,tol", [(T64, 3, 4e-15), (T32, 2, 1e-8), (T32B, 3, 2e-9)])
 def test_one_term_fewer_fails(thresh, degree, tol):
     """Each series truncated one term earlier misses its bound at the
     threshold: the test above can tell a dropped term."""
