@@ -536,7 +536,11 @@ __device__ __forceinline__ void group64_fm(unsigned fm, const Rec64 *tile, const
 // (64-target group, 64-source group) whose bounding boxes are at least
 // sqrt(T_FAR64) apart, the source group's no wider than sqrt(T_EXT64), runs in
 // the far form by moments: the groups are aligned to 64 particles of the job,
-// so the choice is the same for every variant.
+// so the choice is the same for every variant.  Occupancy: one target per
+// thread is held to three waves per SIMD (168 VGPRs; the moments took the
+// 256 x 1, unroll 4 variant to 242, two waves, without the bound), two
+// targets to two (256; at 312 the unroll-4 variant ran one).  Four waves for
+// one target (128 VGPRs, before the moments) measured the same as three.
 template <int KIND, int TB, int TPT, int UNR>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TPT == 1 ? 3 : 2))) void pair64_kernel(const JobDev *__restrict__ jobs,
                                                     const Item *__restrict__ items,
