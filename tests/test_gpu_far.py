@@ -266,22 +266,24 @@ def test_far_lattice_away_from_origin():
     print("  " + "; ".join(out), flush=True)
 
 
+@pytest.mark.parametrize("kernel", [pa.KERNEL_HIGHORDER, pa.KERNEL_GAUSSIAN])
 @pytest.mark.parametrize("spacing", [2.25, 3.0])
-def test_far_lattice_close_tiles(spacing):
+def test_far_lattice_close_tiles(spacing, kernel):
     """The sparse lattice with neighbouring sub-cubes' boxes 9.1 and 12.2 core
     radii apart: far binary32 tiles whose pairs all have 64 <= t < 128 (the
     far-field form with the u^4 term) or t >= 128, beside near ones, against
-    the oracle in both binary32 schemes and binary64."""
+    the oracle in both binary32 schemes and binary64, for both kernels."""
     w = sparse_lattice(spacing=spacing)
     nu = 1e-3
-    print(f"case far lattice spacing {spacing} N={w['n']}", flush=True)
-    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL)
-    v64, d64 = _rates(w, nu, pa.FP64)
+    print(f"case far lattice spacing {spacing} kernel={kernel} N={w['n']}", flush=True)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL,
+                       kind=KIND[kernel])
+    v64, d64 = _rates(w, nu, pa.FP64, kernel)
     e64v, e64d = per_particle_rel(v64, rv), per_particle_rel(d64, rd)
     out = [f"binary64 {e64v:.2e} {e64d:.2e}"]
     assert e64v < TOL64 and e64d < TOL64, (e64v, e64d)
     for sch in (pa.SCHEME_SYMMETRIC, pa.SCHEME_ONESIDED):
-        v, d = _rates(w, nu, pa.FP32C, scheme=sch)
+        v, d = _rates(w, nu, pa.FP32C, kernel, scheme=sch)
         e = (rel_rms(v, rv), rel_rms(d, rd))
         out.append(f"binary32 scheme {sch}: {e[0]:.2e} {e[1]:.2e}")
         assert e[0] < TOL32 and e[1] < TOL32, (sch, e)
