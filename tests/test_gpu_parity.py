@@ -95,6 +95,44 @@ def test_ragged_sizes():
             assert per_particle_rel(vel, rv) < TOL64 and per_particle_rel(dv, rd) < TOL64
 
 
+def test_empty_inputs():
+    """Empty input: the reference cannot hold an empty wake (wake.cpp:10-21
+    allocates its matrices with gsl_matrix_calloc, which rejects a zero
+    dimension), so a wake or batch job of no particles is refused with
+    PW_EINVAL and the context stays usable; a solve before any wake is
+    refused the same way; appending no particles changes nothing."""
+    from pawan_amd import _lib
+    print("case empty inputs", flush=True)
+    z3, z1 = np.zeros((0, 3)), np.zeros(0)
+    S = pa.Interaction(nu=1e-3)
+    with pytest.raises(pa.PawanError) as e:
+        S.solve()
+    assert e.value.code == _lib.PW_EINVAL
+    with pytest.raises(pa.PawanError) as e:
+        S.addWake(pa.Wake(z3, z3, z1, z1))
+    assert e.value.code == _lib.PW_EINVAL
+    w = gen.vortex_ring(33, 7, R=1.0, core=0.1, seed=5)
+    S.addWake(pa.Wake.from_dict(w))
+    assert S._size == 6 * w["n"]
+    S.solve()
+    r0 = S.getRates().copy()
+    S.appendParticles(0, z3, z3, z1, z1)
+    assert S._size == 6 * w["n"]
+    S.solve()
+    r1 = S.getRates()
+    S.close()
+    assert np.array_equal(r0, r1)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], 1e-3, mode=orc.SERIAL)
+    n = w["n"]
+    assert per_particle_rel(r0[:3 * n].reshape(n, 3), rv) < TOL64
+    assert per_particle_rel(r0[3 * n:].reshape(n, 3), rd) < TOL64
+    B = pa.Batch()
+    with pytest.raises(pa.PawanError) as e:
+        B.add({"pos": z3, "vort": z3, "radius": z1, "volume": z1}, 1e-3, 0.01)
+    assert e.value.code == _lib.PW_EINVAL
+    B.close()
+
+
 def test_rk4_trajectory_single_ring():
     """Config 1 for 10 RK4 steps (dt = 0.01) against the oracle's rk4.cpp."""
     w = gen.config_single_ring()
