@@ -205,9 +205,10 @@ def test_far_onesided_variants_bitwise_identical():
                 assert np.array_equal(r, ref), (block, tpt, unroll)
 
 
+@pytest.mark.parametrize("kernel", [pa.KERNEL_HIGHORDER, pa.KERNEL_GAUSSIAN])
 @pytest.mark.parametrize("offset", [0.0, 2.0 ** 20])
 @pytest.mark.parametrize("length", [120.0, 136.0])
-def test_far_wide_source_group(length, offset):
+def test_far_wide_source_group(length, offset, kernel):
     """A compact 64-particle cluster and a 64-particle line `length` core
     radii long, 40 core radii beyond the cluster's box: the binary64 one-sided
     pass adds a far group by moments only when it is at most 128 core radii
@@ -217,7 +218,7 @@ def test_far_wide_source_group(length, offset):
     also placed 2^20 core radii from the origin, where the moments about each
     group's first particle keep the same bound (sigma = 1/16, so that the
     records' coordinates in units of sigma are exact there, as the oracle's
-    are).  Both against the oracle."""
+    are).  Both kernels, against the oracle."""
     core = 1.0 / 16.0
     u = gen.uniform(23, 12 * 64).reshape(128, 6)
     a = 0.5 * core * u[:64, :3]
@@ -231,9 +232,11 @@ def test_far_wide_source_group(length, offset):
     w = {"pos": np.ascontiguousarray(pos), "vort": np.ascontiguousarray(vort),
          "radius": np.full(n, core), "volume": np.full(n, 4.0 / 3.0 * np.pi * core ** 3), "n": n}
     nu = 1e-3
-    print(f"case far wide source group length={length} offset={offset}", flush=True)
-    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL)
-    v64, d64 = _rates(w, nu, pa.FP64)
+    print(f"case far wide source group length={length} offset={offset} kernel={kernel}",
+          flush=True)
+    rv, rd = orc.rates(w["pos"], w["vort"], w["radius"], w["volume"], nu, mode=orc.SERIAL,
+                       kind=KIND[kernel])
+    v64, d64 = _rates(w, nu, pa.FP64, kernel)
     e64v, e64d = per_particle_rel(v64, rv), per_particle_rel(d64, rd)
     # the line's velocities come from its own pairs and from the cluster
     # (a far source group); the cluster's from itself and from the line
